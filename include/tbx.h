@@ -1,0 +1,188 @@
+/*
+ * tbx.h: C ABI of the MI355X semi-Lagrangian advection engine.
+ *
+ * The engine replaces, for one MPI rank's whole tree on one GPU, the hot path
+ * of arashb/tbslas: the RK2 backward-trajectory solve (src/semilag) and the
+ * per-point octree leaf location plus Chebyshev interpolation of velocity
+ * and concentration at departure points (src/tree).  The header-only C++
+ * shim include/tbx/tbslas.hpp restores the reference's functor/step
+ * templates over this ABI.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (TBX_OK = 0).  tbx_last_error()
+ *    returns the calling thread's message for the last failure.  No C++
+ *    exception crosses the ABI.
+ *  - Point arrays are AoS xyz FP64 [n][3]; field values are point-major
+ *    [n][dof] (the reference's functor layout, tree_functor.h:376-383).
+ *  - Pointers passed to compute calls are device pointers (hipMalloc or any
+ *    allocator of device memory) owned by the caller.  The tree handle owns
+ *    device copies of the tree.
+ *  - Calls are ordered on the given stream (NULL = the null stream).  A tree
+ *    handle is externally synchronised: calls that use one handle on two
+ *    host threads at once race.
+ *  - Counts are int64.  Leaf indices are int32 positions in Morton order.
+ *
+ * Leaf layout (PVFMM Cheb_Node, kept so trees move between the two codes):
+ *  leaves are listed in Morton order (PVFMM postorder over non-ghost leaves);
+ *  each has an integer anchor (x,y,z) in units of its own edge 2^-depth and a
+ *  depth; coefficients per leaf are [dof][Nc], Nc = (q+1)(q+2)(q+3)/6, each a
+ *  packed triangle in order i(z) -> j(y) -> k(x), i+j+k <= q, basis
+ *  T_k(x) T_j(y) T_i(z) on the leaf mapped to [-1,1]^3 (tree_functor.h:256-266).
+ */
+#ifndef TBX_H_
+#define TBX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tbx_tree_s tbx_tree;
+typedef void *tbx_stream; /* hipStream_t */
+
+enum {
+  TBX_OK = 0,
+  TBX_EINVAL = 1,      /* bad argument                                  */
+  TBX_ENOMEM = 2,      /* host or device allocation failed              */
+  TBX_EORDER = 3,      /* leaves not strictly in Morton order           */
+  TBX_ECOVER = 4,      /* leaves overlap or leave a gap in [0,1)^3       */
+  TBX_EHIP = 5,        /* HIP runtime or library error                  */
+  TBX_EOVERFLOW = 6,   /* size exceeds an int64/int32 limit             */
+  TBX_EUNSUPPORTED = 7 /* degree or combination this build does not cover */
+};
+
+enum { TBX_BC_FREESPACE = 0, TBX_BC_PERIODIC = 1 }; /* sim_config.h:59 bc */
+
+/* Arithmetic of the coefficient contraction (flags, low byte). */
+#define TBX_FP64 0x0u  /* FP64, fused multiply-add                      */
+#define TBX_EXACT 0x1u /* FP64, the reference's order with separate     */
+                       /* multiply and add (vec_eval, tree_functor.h:56-76); */
+                       /* bit-identical to the CPU path                  */
+#define TBX_FP32 0x2u  /* FP32 coefficients and accumulation; point      */
+                       /* coordinates, RK2 updates and keys stay FP64    */
+#define TBX_MODE_MASK 0xFu
+/* EvalTree wraps the caller's points in place under a periodic bc
+ * (tree_functor.h:442-449, const_cast at :803).  tbx_eval does so only when
+ * this flag is set. */
+#define TBX_WRITE_BACK_WRAP 0x10u
+
+/* Largest Chebyshev degree the kernels are built for. */
+#define TBX_MAX_Q 16
+/* Deepest leaf the engine's Morton keys resolve. */
+#define TBX_MAX_DEPTH 21
+
+/* Host-side description of one tree. */
+typedef struct {
+  int64_t num_leaves;
+  const uint32_t *anchor; /* [L][3] */
+  const uint8_t *depth;   /* [L]    */
+  int32_t q;              /* Chebyshev degree, 1..TBX_MAX_Q             */
+  int32_t dof;            /* field components per point, >= 1           */
+  const double *coeff;    /* host [L][dof][Nc], or NULL for zeros       */
+} tbx_tree_desc;
+
+/* Scalar time modulation g(t) of a steady velocity tree, v(x,t) = g(t) V(x);
+ * NULL means steady.  Evaluated on the host once per RK stage.  It takes the
+ * role of the time argument of the functor concept (traj.inc:33, 40), which
+ * NodeFieldFunctor ignores (tree_functor.h:808-811). */
+typedef double (*tbx_tfac_fn)(double t, void *ctx);
+
+/* ---- tree handle: replaces the pvfmm::MPI_Tree<Cheb_Node<double>> that
+ *      NodeFieldFunctor points at (tree_functor.h:796, 814) -------------- */
+int tbx_tree_create(const tbx_tree_desc *desc, tbx_tree **out);
+int tbx_tree_destroy(tbx_tree *t);
+int tbx_tree_info(const tbx_tree *t, int64_t *num_leaves, int32_t *q,
+                  int32_t *dof, int32_t *max_depth);
+/* Replace / read the coefficients ([L][dof][Nc] FP64).  *_device selects
+ * whether the buffer is device (1) or host (0) memory. */
+int tbx_tree_set_coeffs(tbx_tree *t, const double *coeff, int on_device,
+                        tbx_stream s);
+int tbx_tree_get_coeffs(const tbx_tree *t, double *coeff, int on_device,
+                        tbx_stream s);
+
+/* ---- point location: EvalTree's periodic wrap + Morton keys + sort +
+ *      lower_bound partition (tree_functor.h:442-489, :190-198).  leaf[i] is
+ *      the leaf EvalTree evaluates point i in.  pts are not modified. ---- */
+int tbx_locate(const tbx_tree *t, const double *pts, int64_t n, int bc,
+               int32_t *leaf, tbx_stream s);
+
+/* ---- EvalTree (tree_functor.h:397-791) with EvalNodesLocal (:155-395) and
+ *      vec_eval (:27-84): the NodeFieldFunctor::operator() body (:800-806).
+ *      out: [n][dof]. ------------------------------------------------------ */
+int tbx_eval(const tbx_tree *t, double *pts, int64_t n, int bc, uint32_t flags,
+             double *out, tbx_stream s);
+
+/* ---- ComputeTrajRK2 / IntegrateRK2 (semilag/traj.inc:20-68) through the
+ *      velocity tree's functor.  xsol may alias xinit (the reference runs in
+ *      place, traj.inc:64).  Like the reference, the first stage of every
+ *      substep wraps xsol in place under a periodic bc. ------------------- */
+int tbx_traj_rk2(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
+                 const double *xinit, int64_t n, double tinit, double tfinal,
+                 int num_rk_step, int bc, uint32_t flags, double *xsol,
+                 tbx_stream s);
+
+/* ---- SolveSemilagRK2 (semilag/semilag.inc:26-45): departure points of pts
+ *      over [timestep*dt, timestep*dt - dt], then conc at them.
+ *      vals: [n][conc dof]. ------------------------------------------------ */
+int tbx_semilag_rk2(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
+                    const tbx_tree *conc, const double *pts, int64_t n,
+                    int timestep, double dt, int num_rk_step, int bc,
+                    uint32_t flags, double *vals, tbx_stream s);
+
+/* ---- CollectChebTreeGridPoints (tree_utils.h:442-498): out [L*P][3],
+ *      P = (q+1)^3 stretched nodes per leaf, x fastest (cheb.h:41-67). ---- */
+int tbx_collect_grid_points(const tbx_tree *t, double *out, tbx_stream s);
+
+/* ---- SetTreeGridValues (tree_utils.h:500-552): refit every leaf from
+ *      values at its grid points, vals [L][dof][P] (dof-major per leaf; the
+ *      reference's in-situ call passes point-major values, which agrees for
+ *      dof = 1 only).  coeff = vals * pinv(B)^T (cheb.h:166-196). -------- */
+int tbx_set_grid_values(tbx_tree *t, const double *vals, tbx_stream s);
+
+/* ---- SolveSemilagInSitu (tree/tree_semilag.h:92-135): one fused step.
+ *      Arrival points are generated on the fly, RK2 runs through vel, conc
+ *      is evaluated at the departure points and refit in place.  vel and
+ *      conc must share q; up to 16 RK substeps.  With a conc dof > 1 each
+ *      component is an independent tracer carried by the same trajectories.
+ *      dep, if not NULL, receives the departure points [L*P][3]. --------- */
+int tbx_semilag_insitu(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
+                       tbx_tree *conc, int timestep, double dt,
+                       int num_rk_step, int bc, uint32_t flags, double *dep,
+                       tbx_stream s);
+
+/* ---- host helpers (setup; no device work) ----------------------------- */
+/* new_nodes (cheb.h:41-58), x[q+1]. */
+int tbx_new_nodes(int q, double *x);
+/* GetPt2CoeffMatrix (cheb.h:166-196): M[P][Nc] = pinv(B)^T. */
+int tbx_pt2coeff_matrix(int q, double *M);
+/* Uniform tree of the given depth in Morton order: anchor [8^depth][3],
+ * depth [8^depth]. */
+int tbx_uniform_leaves(int depth, uint32_t *anchor, uint8_t *dep);
+/* First-kind Chebyshev sample points of each leaf, out [L][(q+1)^3][3],
+ * x fastest: where an initial field is sampled for tbx_fit_leaves. */
+int tbx_first_kind_points(int64_t num_leaves, const uint32_t *anchor,
+                          const uint8_t *depth, int q, double *out);
+/* Initial fit (the cheb_approx role in InitTree, tree_utils.h:132-178):
+ * vals [L][(q+1)^3][dof] point-major at tbx_first_kind_points ->
+ * coeff [L][dof][Nc]. */
+int tbx_fit_leaves(int64_t num_leaves, int q, int dof, const double *vals,
+                   double *coeff);
+/* Largest top-shell coefficient sum per leaf, max over dof of
+ * sum_{i+j+k=q} |c_ijk|: the refinement criterion of tbx_refine_*. */
+int tbx_tail_norm(int64_t num_leaves, int q, int dof, const double *coeff,
+                  double *tail);
+/* 2:1 balance (the Balance21 role, tree_utils.h:121-123): splits leaves
+ * until face-, edge- and corner-adjacent leaves differ by at most one level
+ * (wrapping across the domain when periodic).  In/out arrays hold up to
+ * capacity leaves; *num_leaves is updated. */
+int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
+                  uint8_t *depth, int bc);
+
+const char *tbx_last_error(void);
+const char *tbx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TBX_H_ */

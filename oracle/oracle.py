@@ -1,0 +1,230 @@
+"""ctypes binding of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker / CPU baseline.  The engine
+(tbslas_amd, libtbx.so) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+BC_FREESPACE = 0
+BC_PERIODIC = 1
+MAXD = 30
+
+TFAC_FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+
+
+class OrcTree(C.Structure):
+    _fields_ = [
+        ("num_leaves", C.c_int64),
+        ("anchor", C.c_void_p),
+        ("depth", C.c_void_p),
+        ("q", C.c_int32),
+        ("dof", C.c_int32),
+        ("coeff", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        sig = {
+            "orc_ncoef": (C.c_int, [C.c_int]),
+            "orc_new_nodes": (None, [C.c_int, P]),
+            "orc_cheb_nodes_first_kind": (None, [C.c_int, P]),
+            "orc_cheb_poly": (None, [C.c_int, P, C.c_int64, P]),
+            "orc_locate": (C.c_int, [P, P, C.c_int64, C.c_int, P]),
+            "orc_eval_tree": (C.c_int, [P, P, C.c_int64, C.c_int, P]),
+            "orc_traj_rk2": (C.c_int, [P, TFAC_FN, P, P, C.c_int64, C.c_double, C.c_double,
+                                       C.c_int, C.c_int, P]),
+            "orc_semilag_rk2": (C.c_int, [P, TFAC_FN, P, P, P, C.c_int64, C.c_int, C.c_double,
+                                          C.c_int, C.c_int, P]),
+            "orc_collect_grid_points": (C.c_int, [P, P]),
+            "orc_pt2coeff": (C.c_int, [C.c_int, P]),
+            "orc_set_grid_values": (C.c_int, [P, P, P]),
+            "orc_semilag_insitu": (C.c_int, [P, TFAC_FN, P, P, P, C.c_int, C.c_double, C.c_int,
+                                             C.c_int]),
+            "orc_cheb_approx": (C.c_int, [C.c_int, C.c_int, P, P]),
+            "orc_fast_interp": (C.c_int, [P, C.c_int, C.c_int, P, C.c_int64, P]),
+            "orc_cubic1d": (C.c_double, [C.c_double, P, P]),
+            "orc_field_set_eval": (C.c_int, [P, P, P, C.c_int64, C.c_double, C.c_int, P]),
+            "orc_field_extrap_eval": (C.c_int, [P, P, P, C.c_int64, C.c_int, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with code {rc}")
+
+
+def ncoef(q):
+    return (q + 1) * (q + 2) * (q + 3) // 6
+
+
+class Tree:
+    """Host tree for the oracle; keeps its arrays alive."""
+
+    def __init__(self, anchor, depth, q, dof, coeff=None):
+        self.anchor = np.ascontiguousarray(anchor, dtype=np.uint32).reshape(-1, 3)
+        self.depth = np.ascontiguousarray(depth, dtype=np.uint8).reshape(-1)
+        self.q, self.dof = int(q), int(dof)
+        self.L = len(self.depth)
+        self.nc = ncoef(self.q)
+        self.P = (self.q + 1) ** 3
+        if coeff is None:
+            coeff = np.zeros((self.L, self.dof, self.nc))
+        self.coeff = np.ascontiguousarray(coeff, dtype=np.float64).reshape(self.L, self.dof,
+                                                                           self.nc).copy()
+        self.s = OrcTree(self.L, _p(self.anchor), _p(self.depth), self.q, self.dof,
+                         _p(self.coeff))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _tfac(g):
+    if g is None:
+        return TFAC_FN()
+    return TFAC_FN(lambda t, ctx: float(g(t)))
+
+
+def new_nodes(q):
+    x = np.empty(q + 1)
+    lib().orc_new_nodes(q, _p(x))
+    return x
+
+
+def cheb_nodes_first_kind(q):
+    x = np.empty(q + 1)
+    lib().orc_cheb_nodes_first_kind(q, _p(x))
+    return x
+
+
+def cheb_poly(q, x):
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    out = np.empty((q + 1, x.size))
+    lib().orc_cheb_poly(q, _p(x), x.size, _p(out))
+    return out
+
+
+def locate(tree, pts, bc=BC_FREESPACE):
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    out = np.empty(len(pts), dtype=np.int32)
+    _check(lib().orc_locate(tree.ref, _p(pts), len(pts), bc, _p(out)), "locate")
+    return out
+
+
+def eval_tree(tree, pts, bc=BC_FREESPACE, inplace=False):
+    """EvalTree; returns (values [n,dof], wrapped points)."""
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    if not inplace:
+        p = p.copy()
+    out = np.empty((len(p), tree.dof))
+    _check(lib().orc_eval_tree(tree.ref, _p(p), len(p), bc, _p(out)), "eval_tree")
+    return out, p
+
+
+def traj_rk2(vel, xinit, tinit, tfinal, num_rk_step, bc=BC_FREESPACE, g=None):
+    x = np.ascontiguousarray(xinit, dtype=np.float64).reshape(-1, 3)
+    out = np.empty_like(x)
+    cb = _tfac(g)
+    _check(lib().orc_traj_rk2(vel.ref, cb, None, _p(x), len(x), tinit, tfinal, num_rk_step, bc,
+                              _p(out)), "traj_rk2")
+    return out
+
+
+def semilag_rk2(vel, conc, pts, timestep, dt, num_rk_step=1, bc=BC_FREESPACE, g=None):
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    out = np.empty((len(p), conc.dof))
+    cb = _tfac(g)
+    _check(lib().orc_semilag_rk2(vel.ref, cb, None, conc.ref, _p(p), len(p), timestep, dt,
+                                 num_rk_step, bc, _p(out)), "semilag_rk2")
+    return out
+
+
+def collect_grid_points(tree):
+    out = np.empty((tree.L * tree.P, 3))
+    _check(lib().orc_collect_grid_points(tree.ref, _p(out)), "collect_grid_points")
+    return out
+
+
+def pt2coeff(q):
+    M = np.empty(((q + 1) ** 3, ncoef(q)))
+    _check(lib().orc_pt2coeff(q, _p(M)), "pt2coeff")
+    return M
+
+
+def set_grid_values(tree, M, vals):
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    _check(lib().orc_set_grid_values(tree.ref, _p(np.ascontiguousarray(M)), _p(v)),
+           "set_grid_values")
+
+
+def semilag_insitu(vel, conc, M, timestep, dt, num_rk_step=1, bc=BC_FREESPACE, g=None):
+    cb = _tfac(g)
+    _check(lib().orc_semilag_insitu(vel.ref, cb, None, conc.ref, _p(np.ascontiguousarray(M)),
+                                    timestep, dt, num_rk_step, bc), "semilag_insitu")
+
+
+def cheb_approx(q, dof, vals):
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    out = np.empty((dof, ncoef(q)))
+    _check(lib().orc_cheb_approx(q, dof, _p(v), _p(out)), "cheb_approx")
+    return out
+
+
+def fast_interp(grid, dof, N, pts):
+    g = np.ascontiguousarray(grid, dtype=np.float64)
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    out = np.empty((len(p), dof))
+    _check(lib().orc_fast_interp(_p(g), dof, N, _p(p), len(p), _p(out)), "fast_interp")
+    return out
+
+
+def cubic1d(x, xx, pp):
+    xx = np.ascontiguousarray(xx, dtype=np.float64)
+    pp = np.ascontiguousarray(pp, dtype=np.float64)
+    return lib().orc_cubic1d(x, _p(xx), _p(pp))
+
+
+def field_set_eval(trees, tt, pts, t, bc=BC_FREESPACE):
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3).copy()
+    arr = (C.c_void_p * 4)(*[C.addressof(tr.s) for tr in trees])
+    tt = np.ascontiguousarray(tt, dtype=np.float64)
+    out = np.empty((len(p), trees[0].dof))
+    _check(lib().orc_field_set_eval(arr, _p(tt), _p(p), len(p), t, bc, _p(out)),
+           "field_set_eval")
+    return out
+
+
+def field_extrap_eval(tp, tc, pts, bc=BC_FREESPACE):
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3).copy()
+    out = np.empty((len(p), tc.dof))
+    _check(lib().orc_field_extrap_eval(tp.ref, tc.ref, _p(p), len(p), bc, _p(out)),
+           "field_extrap_eval")
+    return out

@@ -1,0 +1,309 @@
+// Device building blocks shared by the HIP kernels of libtbx (gfx950).
+//
+// Execution model of the point kernels: one departure point per lane.  A
+// lane finds its leaf (Morton key -> coarse-cell directory -> bounded binary
+// search), maps the point to the leaf's [-1,1]^3 and builds T_0..T_q per
+// axis in registers.  The coefficient contraction then runs as a waterfall
+// over the distinct leaves present in the wavefront: the first active lane's
+// leaf is broadcast (readfirstlane), every lane in that leaf contracts with
+// that leaf's coefficients, which are wave-uniform and therefore fetched by
+// scalar loads into SGPRs (one fetch feeds 64 lanes), and the loop repeats
+// for the remaining lanes.  Points generated leaf by leaf, or sorted by leaf,
+// keep a wavefront inside one or two leaves, so the waterfall adds little.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tbx_internal.h"
+
+namespace tbx {
+
+template <int MODE>
+struct ModeType {
+  using T = double;
+};
+template <>
+struct ModeType<kFp32> {
+  using T = float;
+};
+
+// Coefficient read path: address space 4 (constant) with a wave-uniform
+// address compiles to s_load_dwordx{2..16}; the data is read-only for the
+// whole launch.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) {
+  return (cptr<T>)(p);
+}
+
+// acc + a*b.  kExact64 keeps the reference's separate multiply and add
+// (vec_eval: add_intrin(w, mul_intrin(px, c)), tree_functor.h:63-76);
+// the other modes fuse.
+template <int MODE, typename T>
+__device__ __forceinline__ T madd(T a, T b, T acc) {
+  if constexpr (MODE == kExact64) {
+    return acc + a * b;
+  } else if constexpr (sizeof(T) == 4) {
+    return __builtin_fmaf(a, b, acc);
+  } else {
+    return __builtin_fma(a, b, acc);
+  }
+}
+
+// Periodic wrap of one coordinate (tree_functor.h:442-449).
+__device__ __forceinline__ double wrap1(double c) {
+  if (c < 0.0) c = c + 1.0;
+  if (c >= 1.0) c = c - 1.0;
+  return c;
+}
+
+// Integer cell of a coordinate at 21 bits.  floor(c * 2^21) is exact for a
+// double in [0,1).  c == 1.0 (and anything above) takes the last cell: this
+// is the reference's "c - 2^-MAX_DEPTH" shift for c == 1.0 under a
+// non-periodic bc (tree_functor.h:464-479), extended to a clamp outside the
+// domain; negative and NaN coordinates take cell 0.
+__device__ __forceinline__ uint32_t cell21(double c) {
+  if (!(c >= 0.0)) return 0u;
+  const double f = floor(c * kKeyScale);
+  return f > kKeyScale - 1.0 ? (uint32_t)(kKeyScale - 1.0) : (uint32_t)f;
+}
+
+// Spread the low 21 bits of v to every third bit.
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1fffffull;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+__device__ __forceinline__ uint64_t point_key(double x, double y, double z) {
+  return spread3(cell21(x)) | (spread3(cell21(y)) << 1) |
+         (spread3(cell21(z)) << 2);
+}
+
+// Index of the leaf whose key range holds key k: the last leaf with
+// key <= k.  The directory narrows the search to the leaves overlapping the
+// point's level-g cell; for trees no deeper than g it is a single lookup.
+__device__ __forceinline__ int locate_key(const TreeView &t, uint64_t k) {
+  const uint32_t c = (uint32_t)(k >> t.dir_shift);
+  int lo = t.dir[c];
+  int hi = t.dir[c + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.key[mid] <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int locate(const TreeView &t, double x, double y,
+                                      double z) {
+  return locate_key(t, point_key(x, y, z));
+}
+
+// Map a point into leaf coordinates: xi = ((x - c) * 2.0) * s - 1.0 with
+// c the leaf corner and s = 2^depth (tree_functor.h:283-295).
+__device__ __forceinline__ void leaf_xi(const TreeView &t, int leaf, double x,
+                                        double y, double z, double &ex,
+                                        double &ey, double &ez) {
+  const uint4 inf = t.info[leaf];
+  const double s = (double)(1u << inf.w);
+  const double len = 1.0 / s;
+  const double cx = (double)inf.x * len;
+  const double cy = (double)inf.y * len;
+  const double cz = (double)inf.z * len;
+  ex = (x - cx) * 2.0 * s - 1.0;
+  ey = (y - cy) * 2.0 * s - 1.0;
+  ez = (z - cz) * 2.0 * s - 1.0;
+}
+
+// T_0..T_{D-1}(x) by T_{k+1} = (2x) T_k - T_{k-1}, evaluated in FP64 and
+// stored as T; |x| > 1 (or NaN) gives all zeros (pvfmm::cheb_poly, called at
+// tree_functor.h:328-330).
+template <int D, typename T>
+__device__ __forceinline__ void cheb_poly(double x, T *p) {
+  const bool in = fabs(x) <= 1.0;
+  const double xx = in ? x : 0.0;
+  double y0 = in ? 1.0 : 0.0;
+  p[0] = (T)y0;
+  if constexpr (D > 1) p[1] = (T)xx;
+  double y1 = xx;
+#pragma unroll
+  for (int k = 2; k < D; ++k) {
+    const double y2 = 2.0 * xx * y1 - y0;
+    p[k] = (T)y2;
+    y0 = y1;
+    y1 = y2;
+  }
+}
+
+// Triangular tensor contraction of one leaf and one component
+// (vec_eval, tree_functor.h:38-77): u = sum_i pz_i (sum_j py_j
+// (sum_k px_k c_ijk)), i+j+k < D, coefficients in packed order.
+// c must be wave-uniform.
+template <int D, int MODE, typename T>
+__device__ __forceinline__ T contract(const T *cg, const T *px, const T *py,
+                                      const T *pz) {
+  const cptr<T> c = as_const(cg);
+  T u = T(0);
+  int idx = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    T v = T(0);
+#pragma unroll
+    for (int j = 0; i + j < D; ++j) {
+      T w = T(0);
+#pragma unroll
+      for (int k = 0; i + j + k < D; ++k) {
+        w = madd<MODE>(px[k], c[idx], w);
+        ++idx;
+      }
+      v = madd<MODE>(py[j], w, v);
+    }
+    u = madd<MODE>(pz[i], v, u);
+  }
+  return u;
+}
+
+template <int D>
+constexpr int ncoef() {
+  return D * (D + 1) * (D + 2) / 6;
+}
+
+template <int MODE>
+__device__ __forceinline__ const typename ModeType<MODE>::T *coeff_base(
+    const TreeView &t) {
+  if constexpr (MODE == kFp32)
+    return t.coeff32;
+  else
+    return t.coeff;
+}
+
+// Inside a waterfall branch every active lane holds the same leaf.  Reading
+// it again with readfirstlane there (instead of reusing the value compared
+// against) keeps the address wave-uniform: the compiler would otherwise
+// substitute the lane's own, divergent, copy of the equal value.
+__device__ __forceinline__ int uniform_leaf(int leaf) {
+  return __builtin_amdgcn_readfirstlane(leaf);
+}
+
+// Waterfall over the distinct leaves held by the active lanes: body(lf) runs
+// once per distinct leaf, with exactly the lanes holding lf active and lf
+// wave-uniform.  The loop is driven by a ballot mask of pending lanes, which
+// shrinks every pass; a loop that only re-tests readfirstlane(leaf) == leaf
+// looks, to the compiler, like one that either exits at once or never
+// exits, and is folded to a single pass.
+template <typename Body>
+__device__ __forceinline__ void for_each_wave_leaf(int leaf, Body body) {
+  uint64_t pending = __builtin_amdgcn_ballot_w64(true);
+  while (pending) {
+    const int src = __builtin_ctzll(pending);
+    const int lf = __builtin_amdgcn_readlane(leaf, src);
+    const bool mine = (leaf == lf);
+    pending &= ~__builtin_amdgcn_ballot_w64(mine);
+    if (mine) body(uniform_leaf(leaf));
+  }
+}
+
+// Per-lane leaf location and basis values for one point.
+template <int D, int MODE>
+struct PointBasis {
+  using T = typename ModeType<MODE>::T;
+  int leaf;
+  T px[D], py[D], pz[D];
+
+  __device__ __forceinline__ void build(const TreeView &t, double x, double y,
+                                        double z) {
+    leaf = locate(t, x, y, z);
+    double ex, ey, ez;
+    leaf_xi(t, leaf, x, y, z, ex, ey, ez);
+    cheb_poly<D>(ex, px);
+    cheb_poly<D>(ey, py);
+    cheb_poly<D>(ez, pz);
+  }
+};
+
+// Velocity (dof 3) at one point: the waterfall over the wave's leaves.
+template <int D, int MODE>
+__device__ __forceinline__ void eval_vec3(const TreeView &t, double x,
+                                          double y, double z, double &v0,
+                                          double &v1, double &v2) {
+  using T = typename ModeType<MODE>::T;
+  constexpr int NC = ncoef<D>();
+  PointBasis<D, MODE> b;
+  b.build(t, x, y, z);
+  const T *base = coeff_base<MODE>(t);
+  for_each_wave_leaf(b.leaf, [&](int lf) {
+    const T *c = base + (size_t)lf * (3 * NC);
+    v0 = (double)contract<D, MODE>(c, b.px, b.py, b.pz);
+    v1 = (double)contract<D, MODE>(c + NC, b.px, b.py, b.pz);
+    v2 = (double)contract<D, MODE>(c + 2 * NC, b.px, b.py, b.pz);
+  });
+}
+
+// All dof components at one point, each written through store(l, value).
+template <int D, int MODE, typename Store>
+__device__ __forceinline__ void eval_dof(const TreeView &t, double x, double y,
+                                         double z, Store store) {
+  using T = typename ModeType<MODE>::T;
+  constexpr int NC = ncoef<D>();
+  PointBasis<D, MODE> b;
+  b.build(t, x, y, z);
+  const T *base = coeff_base<MODE>(t);
+  const int dof = t.dof;
+  for_each_wave_leaf(b.leaf, [&](int lf) {
+    const T *c = base + (size_t)lf * dof * NC;
+    for (int l = 0; l < dof; ++l)
+      store(l, (double)contract<D, MODE>(c + (size_t)l * NC, b.px, b.py, b.pz));
+  });
+}
+
+// One IntegrateRK2 substep (traj.inc:20-46) of one point, in place.  The
+// first evaluation wraps x itself (EvalTree mutates the caller's array,
+// and ComputeTrajRK2 passes xsol as xinit); the midpoint is wrapped only
+// for its evaluation.
+template <int D, int MODE>
+__device__ __forceinline__ void rk2_substep(const TreeView &vel, int bc,
+                                            double htau, double tau, int scaled,
+                                            double g1, double g2, double &x,
+                                            double &y, double &z) {
+  if (bc == TBX_BC_PERIODIC) {
+    x = wrap1(x);
+    y = wrap1(y);
+    z = wrap1(z);
+  }
+  double v0, v1, v2;
+  eval_vec3<D, MODE>(vel, x, y, z, v0, v1, v2);
+  if (scaled) {
+    v0 = v0 * g1;
+    v1 = v1 * g1;
+    v2 = v2 * g1;
+  }
+  // xtmp = xinit + (0.5 * dt) * vtmp   (traj.inc:36)
+  double hx = x + htau * v0, hy = y + htau * v1, hz = z + htau * v2;
+  if (bc == TBX_BC_PERIODIC) {
+    hx = wrap1(hx);
+    hy = wrap1(hy);
+    hz = wrap1(hz);
+  }
+  eval_vec3<D, MODE>(vel, hx, hy, hz, v0, v1, v2);
+  if (scaled) {
+    v0 = v0 * g2;
+    v1 = v1 * g2;
+    v2 = v2 * g2;
+  }
+  // xsol = xinit + dt * vtmp   (traj.inc:41-42)
+  x = x + tau * v0;
+  y = y + tau * v1;
+  z = z + tau * v2;
+}
+
+}  // namespace tbx

@@ -1,0 +1,874 @@
+// Host side of libtbx: the C ABI of include/tbx.h.
+//
+// Owns the device copies of each tree (Morton keys, leaf info, coarse-cell
+// directory, coefficients and their FP32 mirror), checks arguments, picks
+// the per-degree kernels and runs the refit GEMM.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "tbx.h"
+#include "tbx_internal.h"
+
+using namespace tbx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                   \
+  do {                                                                  \
+    hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess)                                               \
+      return fail(TBX_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                  \
+  } while (0)
+
+inline int ncoef(int q) { return (q + 1) * (q + 2) * (q + 3) / 6; }
+
+unsigned host_threads() {
+  unsigned h = std::thread::hardware_concurrency();
+  if (h == 0) h = 1;
+  return std::min(h, 16u);
+}
+
+// Run f(i) for i in [0, n) on a few host threads.
+template <class F>
+void parallel_for(int64_t n, F f) {
+  const unsigned nt = (unsigned)std::min<int64_t>(host_threads(), std::max<int64_t>(n, 1));
+  if (nt <= 1 || n < 64) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  const int64_t chunk = std::max<int64_t>(1, n / (nt * 8));
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&]() {
+      for (;;) {
+        const int64_t a = next.fetch_add(chunk);
+        if (a >= n) break;
+        const int64_t b = std::min(n, a + chunk);
+        for (int64_t i = a; i < b; ++i) f(i);
+      }
+    });
+  for (auto &x : th) x.join();
+}
+
+uint64_t spread3_host(uint32_t v) {
+  uint64_t x = v & 0x1fffffull;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+uint64_t leaf_key(uint32_t ax, uint32_t ay, uint32_t az, int depth) {
+  const int sh = kKeyBits - depth;
+  return spread3_host(ax << sh) | (spread3_host(ay << sh) << 1) |
+         (spread3_host(az << sh) << 2);
+}
+
+uint32_t compact3(uint64_t k) {
+  uint64_t x = k & 0x1249249249249249ull;
+  x = (x | (x >> 2)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x >> 4)) & 0x100f00f00f00f00full;
+  x = (x | (x >> 8)) & 0x1f0000ff0000ffull;
+  x = (x | (x >> 16)) & 0x1f00000000ffffull;
+  x = (x | (x >> 32)) & 0x1fffffull;
+  return (uint32_t)x;
+}
+
+// T_0..T_q at x by the recurrence the kernels use.
+void cheb_poly_host(int q, double x, double *p) {
+  const bool in = std::fabs(x) <= 1.0;
+  const double xx = in ? x : 0.0;
+  double y0 = in ? 1.0 : 0.0;
+  p[0] = y0;
+  if (q >= 1) p[1] = xx;
+  double y1 = xx;
+  for (int k = 2; k <= q; ++k) {
+    const double y2 = 2.0 * xx * y1 - y0;
+    p[k] = y2;
+    y0 = y1;
+    y1 = y2;
+  }
+}
+
+void new_nodes_host(int q, double *x) {
+  const unsigned d = (unsigned)q + 1;
+  const double scal = 1.0 / std::cos(0.5 * M_PI / d);
+  for (unsigned i = 0; i < d; ++i)
+    x[i] = -std::cos((i + 0.5) * M_PI / d) * scal * 0.5 + 0.5;
+}
+
+// M = pinv(B)^T by Householder QR of B (in long double): with B = Q R,
+// M = Q R^-T.  B[p][m] is basis m (packed i,j,k) at stretched node p mapped
+// to [-1,1] (cheb.h:170-180).
+std::vector<double> build_pt2coeff(int q) {
+  typedef long double R_t;
+  const int d = q + 1, P = d * d * d, nc = ncoef(q);
+  std::vector<double> x(d), xi(d), T((size_t)d * d);
+  new_nodes_host(q, x.data());
+  for (int i = 0; i < d; ++i) xi[i] = x[i] * 2.0 - 1.0;
+  for (int i = 0; i < d; ++i) {
+    std::vector<double> p(d);
+    cheb_poly_host(q, xi[i], p.data());
+    for (int k = 0; k < d; ++k) T[(size_t)k * d + i] = p[k];
+  }
+  // A column-major P x nc
+  std::vector<R_t> A((size_t)P * nc);
+  {
+    int m = 0;
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; i + j < d; ++j)
+        for (int k = 0; i + j + k < d; ++k, ++m) {
+          R_t *col = &A[(size_t)m * P];
+          for (int p = 0; p < P; ++p) {
+            const int ix = p % d, iy = (p / d) % d, iz = p / (d * d);
+            col[p] = (R_t)T[(size_t)k * d + ix] * T[(size_t)j * d + iy] * T[(size_t)i * d + iz];
+          }
+        }
+  }
+  std::vector<R_t> R((size_t)nc * nc, 0.0), V((size_t)P * nc, 0.0), beta(nc);
+  for (int c = 0; c < nc; ++c) {
+    R_t *a = &A[(size_t)c * P];
+    R_t nrm = 0;
+    for (int p = c; p < P; ++p) nrm += a[p] * a[p];
+    nrm = std::sqrt(nrm);
+    const R_t alpha = a[c] > 0 ? -nrm : nrm;
+    R_t *v = &V[(size_t)c * P];
+    for (int p = c; p < P; ++p) v[p] = a[p];
+    v[c] -= alpha;
+    R_t vv = 0;
+    for (int p = c; p < P; ++p) vv += v[p] * v[p];
+    beta[c] = vv > 0 ? 2.0L / vv : 0.0L;
+    R[(size_t)c * nc + c] = alpha;
+    parallel_for(nc - c - 1, [&](int64_t jj) {
+      const int j = c + 1 + (int)jj;
+      R_t *aj = &A[(size_t)j * P];
+      R_t s = 0;
+      for (int p = c; p < P; ++p) s += v[p] * aj[p];
+      s *= beta[c];
+      for (int p = c; p < P; ++p) aj[p] -= s * v[p];
+    });
+    for (int j = c + 1; j < nc; ++j) R[(size_t)c * nc + j] = A[(size_t)j * P + c];
+  }
+  // Q (P x nc, column-major) = H_0 ... H_{nc-1} [I; 0]
+  std::vector<R_t> Q((size_t)P * nc, 0.0);
+  for (int c = 0; c < nc; ++c) Q[(size_t)c * P + c] = 1.0;
+  parallel_for(nc, [&](int64_t col) {
+    R_t *qc = &Q[(size_t)col * P];
+    for (int c = nc - 1; c >= 0; --c) {
+      const R_t *v = &V[(size_t)c * P];
+      R_t s = 0;
+      for (int p = c; p < P; ++p) s += v[p] * qc[p];
+      s *= beta[c];
+      if (s != 0)
+        for (int p = c; p < P; ++p) qc[p] -= s * v[p];
+    }
+  });
+  // row p of M solves R m = Q[p,:]^T
+  std::vector<double> M((size_t)P * nc);
+  parallel_for(P, [&](int64_t p) {
+    std::vector<R_t> r(nc);
+    for (int a = 0; a < nc; ++a) r[a] = Q[(size_t)a * P + p];
+    for (int a = nc - 1; a >= 0; --a) {
+      R_t s = r[a];
+      for (int k = a + 1; k < nc; ++k) s -= R[(size_t)a * nc + k] * r[k];
+      r[a] = s / R[(size_t)a * nc + a];
+    }
+    for (int a = 0; a < nc; ++a) M[(size_t)p * nc + a] = (double)r[a];
+  });
+  return M;
+}
+
+const std::vector<double> &pt2coeff_cached(int q) {
+  static std::mutex mu;
+  static std::map<int, std::vector<double>> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(q);
+  if (it == cache.end()) it = cache.emplace(q, build_pt2coeff(q)).first;
+  return it->second;
+}
+
+}  // namespace
+
+struct tbx_tree_s {
+  int device = 0;
+  int64_t L = 0;
+  int q = 0, dof = 0, nc = 0, max_depth = 0, dir_level = 0;
+  std::vector<uint64_t> h_key;
+  std::vector<uint4> h_info;
+  uint64_t *d_key = nullptr;
+  uint4 *d_info = nullptr;
+  int32_t *d_dir = nullptr;
+  double *d_coeff = nullptr;
+  double *d_coeff_next = nullptr;  // refit target, swapped after the GEMM
+  float *d_coeff32 = nullptr;
+  bool f32_stale = true;
+  double *d_vals = nullptr;  // [L][dof][P] values at the grid points
+  double *d_M = nullptr;     // [P][nc] refit operator
+  rocblas_handle blas = nullptr;
+
+  TreeView view() const {
+    TreeView v{};
+    v.key = d_key;
+    v.info = d_info;
+    v.dir = d_dir;
+    v.coeff = d_coeff;
+    v.coeff32 = d_coeff32;
+    v.L = L;
+    v.dof = dof;
+    v.nc = nc;
+    v.q = q;
+    v.dir_shift = 3 * (kKeyBits - dir_level);
+    return v;
+  }
+  int64_t ncoeff_total() const { return L * dof * nc; }
+  int P() const { return (q + 1) * (q + 1) * (q + 1); }
+};
+
+namespace {
+
+void free_tree(tbx_tree *t) {
+  if (!t) return;
+  hipFree(t->d_key);
+  hipFree(t->d_info);
+  hipFree(t->d_dir);
+  hipFree(t->d_coeff);
+  hipFree(t->d_coeff_next);
+  hipFree(t->d_coeff32);
+  hipFree(t->d_vals);
+  hipFree(t->d_M);
+  if (t->blas) rocblas_destroy_handle(t->blas);
+  delete t;
+}
+
+int ensure_f32(tbx_tree *t, hipStream_t s) {
+  if (!t->d_coeff32) {
+    HIP_TRY(hipMalloc(&t->d_coeff32, sizeof(float) * (size_t)t->ncoeff_total()));
+    t->f32_stale = true;
+  }
+  if (t->f32_stale) {
+    HIP_TRY(launch_to_f32(t->d_coeff, t->d_coeff32, t->ncoeff_total(), s));
+    t->f32_stale = false;
+  }
+  return TBX_OK;
+}
+
+int mode_of(uint32_t flags, int *mode) {
+  const uint32_t m = flags & TBX_MODE_MASK;
+  if (m == TBX_FP64)
+    *mode = kFma64;
+  else if (m == TBX_EXACT)
+    *mode = kExact64;
+  else if (m == TBX_FP32)
+    *mode = kFp32;
+  else
+    return fail(TBX_EINVAL, "unknown arithmetic mode 0x%x", m);
+  return TBX_OK;
+}
+
+int prepare_mode(tbx_tree *t, int mode, hipStream_t s) {
+  if (mode == kFp32) return ensure_f32(t, s);
+  return TBX_OK;
+}
+
+#define TBX_Q_CASES(X) \
+  X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
+hipError_t dispatch_eval(int q, int mode, const TreeView &t, double *pts,
+                         int64_t n, int bc, int wb, double *out, hipStream_t s) {
+  switch (q) {
+#define X(Q) \
+  case Q:    \
+    return launch_eval_q##Q(mode, t, pts, n, bc, wb, out, s);
+    TBX_Q_CASES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t dispatch_traj(int q, int mode, const TreeView &vel, double *x,
+                         int64_t n, const SubArgs &a, hipStream_t s) {
+  switch (q) {
+#define X(Q) \
+  case Q:    \
+    return launch_traj_q##Q(mode, vel, x, n, a, s);
+    TBX_Q_CASES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t dispatch_step(int q, int mode, const TreeView &vel,
+                         const TreeView &conc, const StepArgs &a, double *vals,
+                         double *dep, hipStream_t s) {
+  switch (q) {
+#define X(Q) \
+  case Q:    \
+    return launch_step_q##Q(mode, vel, conc, a, vals, dep, s);
+    TBX_Q_CASES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+int check_points(const void *p, int64_t n, const char *what) {
+  if (n < 0) return fail(TBX_EINVAL, "%s: negative point count", what);
+  if (n > 0 && !p) return fail(TBX_EINVAL, "%s: null buffer", what);
+  if (n > ((int64_t)1 << 40)) return fail(TBX_EOVERFLOW, "%s: %lld points", what, (long long)n);
+  return TBX_OK;
+}
+
+int check_bc(int bc) {
+  if (bc != TBX_BC_FREESPACE && bc != TBX_BC_PERIODIC)
+    return fail(TBX_EINVAL, "unknown boundary condition %d", bc);
+  return TBX_OK;
+}
+
+// Stage factors of one substep at current time tcur (traj.inc:33-40).
+SubArgs sub_args(double tcur, double tau, tbx_tfac_fn g, void *gctx, int bc) {
+  SubArgs a{};
+  a.tau = tau;
+  a.htau = 0.5 * tau;
+  a.scaled = g ? 1 : 0;
+  a.g1 = g ? g(tcur, gctx) : 1.0;
+  a.g2 = g ? g(tcur + 0.5 * tau, gctx) : 1.0;
+  a.bc = bc;
+  return a;
+}
+
+int ensure_refit(tbx_tree *t, hipStream_t s) {
+  const int P = t->P();
+  if (!t->d_M) {
+    const std::vector<double> &M = pt2coeff_cached(t->q);
+    HIP_TRY(hipMalloc(&t->d_M, sizeof(double) * M.size()));
+    HIP_TRY(hipMemcpy(t->d_M, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice));
+  }
+  if (!t->d_coeff_next)
+    HIP_TRY(hipMalloc(&t->d_coeff_next, sizeof(double) * (size_t)t->ncoeff_total()));
+  if (!t->blas) {
+    if (rocblas_create_handle(&t->blas) != rocblas_status_success)
+      return fail(TBX_EHIP, "rocblas_create_handle failed");
+  }
+  if (rocblas_set_stream(t->blas, s) != rocblas_status_success)
+    return fail(TBX_EHIP, "rocblas_set_stream failed");
+  (void)P;
+  return TBX_OK;
+}
+
+// coeff_next[r][m] = sum_p vals[r][p] M[p][m], r over L*dof rows, then swap
+// (SetTreeGridValues GEMM, tree_utils.h:534-546).
+int refit(tbx_tree *t, const double *vals, hipStream_t s) {
+  int rc = ensure_refit(t, s);
+  if (rc) return rc;
+  const int P = t->P(), nc = t->nc;
+  const int64_t rows = t->L * t->dof;
+  if (rows > INT32_MAX) return fail(TBX_EOVERFLOW, "refit: %lld rows", (long long)rows);
+  const double one = 1.0, zero = 0.0;
+  rocblas_status st = rocblas_dgemm(t->blas, rocblas_operation_none, rocblas_operation_none,
+                                    nc, (rocblas_int)rows, P, &one, t->d_M, nc, vals, P,
+                                    &zero, t->d_coeff_next, nc);
+  if (st != rocblas_status_success) return fail(TBX_EHIP, "rocblas_dgemm failed (%d)", (int)st);
+  std::swap(t->d_coeff, t->d_coeff_next);
+  t->f32_stale = true;
+  return TBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tbx_last_error(void) { return g_err.c_str(); }
+const char *tbx_version(void) { return "tbx 0.1 (gfx950)"; }
+
+int tbx_tree_create(const tbx_tree_desc *desc, tbx_tree **out) {
+  if (!desc || !out) return fail(TBX_EINVAL, "tbx_tree_create: null argument");
+  *out = nullptr;
+  const int64_t L = desc->num_leaves;
+  if (L <= 0 || !desc->anchor || !desc->depth)
+    return fail(TBX_EINVAL, "tbx_tree_create: empty leaf list");
+  if (L > INT32_MAX) return fail(TBX_EOVERFLOW, "tbx_tree_create: %lld leaves", (long long)L);
+  if (desc->q < 1 || desc->q > TBX_MAX_Q)
+    return fail(TBX_EUNSUPPORTED, "tbx_tree_create: degree %d outside 1..%d", desc->q, TBX_MAX_Q);
+  if (desc->dof < 1) return fail(TBX_EINVAL, "tbx_tree_create: dof %d", desc->dof);
+  const int nc = ncoef(desc->q);
+  if ((double)L * desc->dof * nc > 4.0e10)
+    return fail(TBX_EOVERFLOW, "tbx_tree_create: coefficient array too large");
+
+  tbx_tree *t = new tbx_tree_s();
+  t->L = L;
+  t->q = desc->q;
+  t->dof = desc->dof;
+  t->nc = nc;
+  t->h_key.resize(L);
+  t->h_info.resize(L);
+  int maxd = 0;
+  for (int64_t j = 0; j < L; ++j) {
+    const int dj = desc->depth[j];
+    const uint32_t ax = desc->anchor[3 * j], ay = desc->anchor[3 * j + 1],
+                   az = desc->anchor[3 * j + 2];
+    if (dj > kKeyBits) {
+      free_tree(t);
+      return fail(TBX_EUNSUPPORTED, "leaf %lld: depth %d > %d", (long long)j, dj, kKeyBits);
+    }
+    const uint32_t lim = 1u << dj;
+    if (ax >= lim || ay >= lim || az >= lim) {
+      free_tree(t);
+      return fail(TBX_EINVAL, "leaf %lld: anchor outside depth-%d grid", (long long)j, dj);
+    }
+    t->h_key[j] = leaf_key(ax, ay, az, dj);
+    t->h_info[j] = make_uint4(ax, ay, az, (uint32_t)dj);
+    maxd = std::max(maxd, dj);
+  }
+  // strictly increasing keys whose ranges tile [0, 8^21) without gaps
+  const unsigned __int128 total = (unsigned __int128)1 << (3 * kKeyBits);
+  if (t->h_key[0] != 0) {
+    free_tree(t);
+    return fail(TBX_ECOVER, "first leaf does not start at the origin");
+  }
+  for (int64_t j = 0; j < L; ++j) {
+    const unsigned __int128 end =
+        (unsigned __int128)t->h_key[j] + ((unsigned __int128)1 << (3 * (kKeyBits - desc->depth[j])));
+    if (j + 1 < L) {
+      if (t->h_key[j + 1] <= t->h_key[j]) {
+        free_tree(t);
+        return fail(TBX_EORDER, "leaves %lld and %lld are not in Morton order", (long long)j,
+                    (long long)(j + 1));
+      }
+      if (end != t->h_key[j + 1]) {
+        free_tree(t);
+        return fail(TBX_ECOVER, "leaf %lld and the next leaf overlap or leave a gap", (long long)j);
+      }
+    } else if (end != total) {
+      free_tree(t);
+      return fail(TBX_ECOVER, "last leaf does not reach the far corner");
+    }
+  }
+  t->max_depth = maxd;
+  hipGetDevice(&t->device);
+  t->dir_level = std::min(maxd, 7);
+  const int64_t ncell = (int64_t)1 << (3 * t->dir_level);
+  std::vector<int32_t> dir(ncell + 1);
+  {
+    const int sh = 3 * (kKeyBits - t->dir_level);
+    int64_t j = 0;
+    for (int64_t c = 0; c < ncell; ++c) {
+      const uint64_t ck = (uint64_t)c << sh;
+      while (j + 1 < L && t->h_key[j + 1] <= ck) ++j;
+      dir[c] = (int32_t)j;
+    }
+    dir[ncell] = (int32_t)(L - 1);
+  }
+  auto bad = [&](hipError_t e, const char *what) {
+    free_tree(t);
+    return fail(e == hipErrorOutOfMemory ? TBX_ENOMEM : TBX_EHIP, "%s: %s", what,
+                hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&t->d_key, sizeof(uint64_t) * L)) != hipSuccess) return bad(e, "hipMalloc");
+  if ((e = hipMalloc(&t->d_info, sizeof(uint4) * L)) != hipSuccess) return bad(e, "hipMalloc");
+  if ((e = hipMalloc(&t->d_dir, sizeof(int32_t) * dir.size())) != hipSuccess) return bad(e, "hipMalloc");
+  if ((e = hipMalloc(&t->d_coeff, sizeof(double) * (size_t)t->ncoeff_total())) != hipSuccess)
+    return bad(e, "hipMalloc");
+  if ((e = hipMemcpy(t->d_key, t->h_key.data(), sizeof(uint64_t) * L, hipMemcpyHostToDevice)) != hipSuccess)
+    return bad(e, "hipMemcpy");
+  if ((e = hipMemcpy(t->d_info, t->h_info.data(), sizeof(uint4) * L, hipMemcpyHostToDevice)) != hipSuccess)
+    return bad(e, "hipMemcpy");
+  if ((e = hipMemcpy(t->d_dir, dir.data(), sizeof(int32_t) * dir.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return bad(e, "hipMemcpy");
+  if (desc->coeff)
+    e = hipMemcpy(t->d_coeff, desc->coeff, sizeof(double) * (size_t)t->ncoeff_total(), hipMemcpyHostToDevice);
+  else
+    e = hipMemset(t->d_coeff, 0, sizeof(double) * (size_t)t->ncoeff_total());
+  if (e != hipSuccess) return bad(e, "coefficient upload");
+  *out = t;
+  return TBX_OK;
+}
+
+int tbx_tree_destroy(tbx_tree *t) {
+  if (!t) return fail(TBX_EINVAL, "tbx_tree_destroy: null tree");
+  hipDeviceSynchronize();
+  free_tree(t);
+  return TBX_OK;
+}
+
+int tbx_tree_info(const tbx_tree *t, int64_t *num_leaves, int32_t *q, int32_t *dof,
+                  int32_t *max_depth) {
+  if (!t) return fail(TBX_EINVAL, "tbx_tree_info: null tree");
+  if (num_leaves) *num_leaves = t->L;
+  if (q) *q = t->q;
+  if (dof) *dof = t->dof;
+  if (max_depth) *max_depth = t->max_depth;
+  return TBX_OK;
+}
+
+int tbx_tree_set_coeffs(tbx_tree *t, const double *coeff, int on_device, tbx_stream s) {
+  if (!t || !coeff) return fail(TBX_EINVAL, "tbx_tree_set_coeffs: null argument");
+  HIP_TRY(hipMemcpyAsync(t->d_coeff, coeff, sizeof(double) * (size_t)t->ncoeff_total(),
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                         (hipStream_t)s));
+  if (!on_device) HIP_TRY(hipStreamSynchronize((hipStream_t)s));
+  t->f32_stale = true;
+  return TBX_OK;
+}
+
+int tbx_tree_get_coeffs(const tbx_tree *t, double *coeff, int on_device, tbx_stream s) {
+  if (!t || !coeff) return fail(TBX_EINVAL, "tbx_tree_get_coeffs: null argument");
+  HIP_TRY(hipMemcpyAsync(coeff, t->d_coeff, sizeof(double) * (size_t)t->ncoeff_total(),
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                         (hipStream_t)s));
+  if (!on_device) HIP_TRY(hipStreamSynchronize((hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_locate(const tbx_tree *t, const double *pts, int64_t n, int bc, int32_t *leaf,
+               tbx_stream s) {
+  if (!t) return fail(TBX_EINVAL, "tbx_locate: null tree");
+  int rc = check_points(pts, n, "tbx_locate");
+  if (rc) return rc;
+  if ((rc = check_bc(bc))) return rc;
+  if (n > 0 && !leaf) return fail(TBX_EINVAL, "tbx_locate: null output");
+  HIP_TRY(launch_locate(t->view(), pts, n, bc, leaf, (hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_eval(const tbx_tree *tc, double *pts, int64_t n, int bc, uint32_t flags, double *out,
+             tbx_stream s) {
+  tbx_tree *t = const_cast<tbx_tree *>(tc);
+  if (!t) return fail(TBX_EINVAL, "tbx_eval: null tree");
+  int rc = check_points(pts, n, "tbx_eval");
+  if (rc) return rc;
+  if ((rc = check_bc(bc))) return rc;
+  if (n > 0 && !out) return fail(TBX_EINVAL, "tbx_eval: null output");
+  int mode;
+  if ((rc = mode_of(flags, &mode))) return rc;
+  if ((rc = prepare_mode(t, mode, (hipStream_t)s))) return rc;
+  HIP_TRY(dispatch_eval(t->q, mode, t->view(), pts, n, bc,
+                        (flags & TBX_WRITE_BACK_WRAP) ? 1 : 0, out, (hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_traj_rk2(const tbx_tree *velc, tbx_tfac_fn g, void *gctx, const double *xinit, int64_t n,
+                 double tinit, double tfinal, int num_rk_step, int bc, uint32_t flags,
+                 double *xsol, tbx_stream s) {
+  tbx_tree *vel = const_cast<tbx_tree *>(velc);
+  if (!vel) return fail(TBX_EINVAL, "tbx_traj_rk2: null tree");
+  if (vel->dof != 3) return fail(TBX_EINVAL, "tbx_traj_rk2: velocity tree has dof %d", vel->dof);
+  int rc = check_points(xinit, n, "tbx_traj_rk2");
+  if (rc) return rc;
+  if ((rc = check_bc(bc))) return rc;
+  if (num_rk_step <= 0) return fail(TBX_EINVAL, "tbx_traj_rk2: num_rk_step %d", num_rk_step);
+  if (n > 0 && !xsol) return fail(TBX_EINVAL, "tbx_traj_rk2: null output");
+  int mode;
+  if ((rc = mode_of(flags, &mode))) return rc;
+  if ((rc = prepare_mode(vel, mode, (hipStream_t)s))) return rc;
+  if (n == 0) return TBX_OK;
+  if (xsol != xinit)
+    HIP_TRY(hipMemcpyAsync(xsol, xinit, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice,
+                           (hipStream_t)s));
+  const double tau = (tfinal - tinit) / num_rk_step;  // traj.inc:55
+  double tcur = tinit;
+  for (int k = 0; k < num_rk_step; ++k) {
+    const SubArgs a = sub_args(tcur, tau, g, gctx, bc);
+    HIP_TRY(dispatch_traj(vel->q, mode, vel->view(), xsol, n, a, (hipStream_t)s));
+    tcur = tcur + tau;  // traj.inc:65
+  }
+  return TBX_OK;
+}
+
+int tbx_semilag_rk2(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, const tbx_tree *concc,
+                    const double *pts, int64_t n, int timestep, double dt, int num_rk_step,
+                    int bc, uint32_t flags, double *vals, tbx_stream s) {
+  tbx_tree *conc = const_cast<tbx_tree *>(concc);
+  if (!vel || !conc) return fail(TBX_EINVAL, "tbx_semilag_rk2: null tree");
+  int rc = check_points(pts, n, "tbx_semilag_rk2");
+  if (rc) return rc;
+  if (n > 0 && !vals) return fail(TBX_EINVAL, "tbx_semilag_rk2: null output");
+  if (n == 0) return TBX_OK;
+  const double tinit = timestep * dt;  // semilag.inc:34-35
+  const double tfinal = tinit - dt;
+  double *dep = nullptr;
+  HIP_TRY(hipMallocAsync((void **)&dep, sizeof(double) * 3 * (size_t)n, (hipStream_t)s));
+  rc = tbx_traj_rk2(vel, g, gctx, pts, n, tinit, tfinal, num_rk_step, bc, flags, dep, s);
+  if (!rc) rc = tbx_eval(conc, dep, n, bc, flags & TBX_MODE_MASK, vals, s);
+  hipFreeAsync(dep, (hipStream_t)s);
+  return rc;
+}
+
+int tbx_collect_grid_points(const tbx_tree *t, double *out, tbx_stream s) {
+  if (!t || !out) return fail(TBX_EINVAL, "tbx_collect_grid_points: null argument");
+  double node[TBX_MAX_Q + 1];
+  new_nodes_host(t->q, node);
+  HIP_TRY(launch_grid_points(t->view(), node, t->q + 1, out, (hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_set_grid_values(tbx_tree *t, const double *vals, tbx_stream s) {
+  if (!t || !vals) return fail(TBX_EINVAL, "tbx_set_grid_values: null argument");
+  return refit(t, vals, (hipStream_t)s);
+}
+
+int tbx_semilag_insitu(const tbx_tree *velc, tbx_tfac_fn g, void *gctx, tbx_tree *conc,
+                       int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
+                       double *dep, tbx_stream s) {
+  tbx_tree *vel = const_cast<tbx_tree *>(velc);
+  if (!vel || !conc) return fail(TBX_EINVAL, "tbx_semilag_insitu: null tree");
+  if (vel->dof != 3) return fail(TBX_EINVAL, "tbx_semilag_insitu: velocity dof %d", vel->dof);
+  if (vel->q != conc->q)
+    return fail(TBX_EUNSUPPORTED, "tbx_semilag_insitu: velocity q %d != concentration q %d",
+                vel->q, conc->q);
+  if (num_rk_step <= 0 || num_rk_step > kMaxSub)
+    return fail(TBX_EUNSUPPORTED, "tbx_semilag_insitu: num_rk_step %d outside 1..%d",
+                num_rk_step, kMaxSub);
+  int rc = check_bc(bc);
+  if (rc) return rc;
+  int mode;
+  if ((rc = mode_of(flags, &mode))) return rc;
+  hipStream_t st = (hipStream_t)s;
+  if ((rc = prepare_mode(vel, mode, st))) return rc;
+  if ((rc = prepare_mode(conc, mode, st))) return rc;
+  const int P = conc->P();
+  if (!conc->d_vals) HIP_TRY(hipMalloc(&conc->d_vals, sizeof(double) * (size_t)conc->L * conc->dof * P));
+
+  StepArgs a{};
+  a.npts = conc->L * P;
+  a.nblocks = (a.npts + kBlock - 1) / kBlock;
+  a.per_xcd = (a.nblocks + 7) / 8;
+  if (8 * a.per_xcd > INT32_MAX) return fail(TBX_EOVERFLOW, "tbx_semilag_insitu: grid too large");
+  a.P = P;
+  a.nsub = num_rk_step;
+  a.bc = bc;
+  a.scaled = g ? 1 : 0;
+  const double tinit = timestep * dt, tfinal = tinit - dt;  // semilag.inc:34-35
+  a.tau = (tfinal - tinit) / num_rk_step;                    // traj.inc:55
+  a.htau = 0.5 * a.tau;
+  double tcur = tinit;
+  for (int k = 0; k < num_rk_step; ++k) {
+    a.g[2 * k] = g ? g(tcur, gctx) : 1.0;
+    a.g[2 * k + 1] = g ? g(tcur + 0.5 * a.tau, gctx) : 1.0;
+    tcur = tcur + a.tau;
+  }
+  new_nodes_host(conc->q, a.node);
+  HIP_TRY(dispatch_step(conc->q, mode, vel->view(), conc->view(), a, conc->d_vals, dep, st));
+  return refit(conc, conc->d_vals, st);
+}
+
+int tbx_new_nodes(int q, double *x) {
+  if (q < 0 || q > 64 || !x) return fail(TBX_EINVAL, "tbx_new_nodes: bad argument");
+  new_nodes_host(q, x);
+  return TBX_OK;
+}
+
+int tbx_pt2coeff_matrix(int q, double *M) {
+  if (q < 1 || q > TBX_MAX_Q || !M) return fail(TBX_EINVAL, "tbx_pt2coeff_matrix: bad argument");
+  const std::vector<double> &src = pt2coeff_cached(q);
+  std::memcpy(M, src.data(), sizeof(double) * src.size());
+  return TBX_OK;
+}
+
+int tbx_uniform_leaves(int depth, uint32_t *anchor, uint8_t *dep) {
+  if (depth < 0 || depth > 10 || !anchor || !dep)
+    return fail(TBX_EINVAL, "tbx_uniform_leaves: depth %d", depth);
+  const int64_t L = (int64_t)1 << (3 * depth);
+  for (int64_t m = 0; m < L; ++m) {
+    anchor[3 * m + 0] = compact3((uint64_t)m);
+    anchor[3 * m + 1] = compact3((uint64_t)m >> 1);
+    anchor[3 * m + 2] = compact3((uint64_t)m >> 2);
+    dep[m] = (uint8_t)depth;
+  }
+  return TBX_OK;
+}
+
+int tbx_first_kind_points(int64_t L, const uint32_t *anchor, const uint8_t *depth, int q,
+                          double *out) {
+  if (L < 0 || q < 0 || q > 64 || (L > 0 && (!anchor || !depth || !out)))
+    return fail(TBX_EINVAL, "tbx_first_kind_points: bad argument");
+  const int d = q + 1, P = d * d * d;
+  std::vector<double> x(d);
+  for (int i = 0; i < d; ++i) x[i] = -std::cos((i + 0.5) * M_PI / d) * 0.5 + 0.5;
+  parallel_for(L, [&](int64_t j) {
+    const double len = std::pow(0.5, depth[j]);
+    const double c0 = anchor[3 * j] * len, c1 = anchor[3 * j + 1] * len, c2 = anchor[3 * j + 2] * len;
+    double *o = out + (size_t)j * P * 3;
+    for (int m = 0; m < P; ++m) {
+      o[3 * m + 0] = c0 + len * x[m % d];
+      o[3 * m + 1] = c1 + len * x[(m / d) % d];
+      o[3 * m + 2] = c2 + len * x[m / (d * d)];
+    }
+  });
+  return TBX_OK;
+}
+
+int tbx_fit_leaves(int64_t L, int q, int dof, const double *vals, double *coeff) {
+  if (L < 0 || q < 0 || q > 64 || dof < 1 || (L > 0 && (!vals || !coeff)))
+    return fail(TBX_EINVAL, "tbx_fit_leaves: bad argument");
+  const int d = q + 1, P = d * d * d, nc = ncoef(q);
+  // C[k][i] = (2/d) T_k(xi_i), halved for k = 0: the discrete Chebyshev
+  // transform at first-kind nodes.
+  std::vector<double> C((size_t)d * d);
+  for (int i = 0; i < d; ++i) {
+    const double xi = (-std::cos((i + 0.5) * M_PI / d) * 0.5 + 0.5) * 2.0 - 1.0;
+    std::vector<double> p(d);
+    cheb_poly_host(q, xi, p.data());
+    for (int k = 0; k < d; ++k) C[(size_t)k * d + i] = p[k] * (2.0 / d) * (k == 0 ? 0.5 : 1.0);
+  }
+  parallel_for(L * dof, [&](int64_t r) {
+    const int64_t j = r / dof;
+    const int l = (int)(r % dof);
+    std::vector<double> a(P), b(P);
+    for (int m = 0; m < P; ++m) a[m] = vals[((size_t)j * P + m) * dof + l];
+    // a[z][y][x] -> b[z][y][kx] -> a[z][ky][kx] -> b[kz][ky][kx]
+    for (int zy = 0; zy < d * d; ++zy)
+      for (int k = 0; k < d; ++k) {
+        double s = 0;
+        for (int i = 0; i < d; ++i) s += C[(size_t)k * d + i] * a[(size_t)zy * d + i];
+        b[(size_t)zy * d + k] = s;
+      }
+    for (int z = 0; z < d; ++z)
+      for (int kx = 0; kx < d; ++kx)
+        for (int k = 0; k < d; ++k) {
+          double s = 0;
+          for (int i = 0; i < d; ++i) s += C[(size_t)k * d + i] * b[((size_t)z * d + i) * d + kx];
+          a[((size_t)z * d + k) * d + kx] = s;
+        }
+    for (int ky = 0; ky < d; ++ky)
+      for (int kx = 0; kx < d; ++kx)
+        for (int k = 0; k < d; ++k) {
+          double s = 0;
+          for (int i = 0; i < d; ++i) s += C[(size_t)k * d + i] * a[((size_t)i * d + ky) * d + kx];
+          b[((size_t)k * d + ky) * d + kx] = s;
+        }
+    double *c = coeff + ((size_t)j * dof + l) * nc;
+    int m = 0;
+    for (int i = 0; i < d; ++i)
+      for (int jj = 0; i + jj < d; ++jj)
+        for (int k = 0; i + jj + k < d; ++k) c[m++] = b[((size_t)i * d + jj) * d + k];
+  });
+  return TBX_OK;
+}
+
+int tbx_tail_norm(int64_t L, int q, int dof, const double *coeff, double *tail) {
+  if (L < 0 || q < 0 || dof < 1 || (L > 0 && (!coeff || !tail)))
+    return fail(TBX_EINVAL, "tbx_tail_norm: bad argument");
+  const int d = q + 1, nc = ncoef(q);
+  std::vector<int> shell;
+  {
+    int m = 0;
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; i + j < d; ++j)
+        for (int k = 0; i + j + k < d; ++k, ++m)
+          if (i + j + k == q) shell.push_back(m);
+  }
+  parallel_for(L, [&](int64_t j) {
+    double best = 0;
+    for (int l = 0; l < dof; ++l) {
+      const double *c = coeff + ((size_t)j * dof + l) * nc;
+      double s = 0;
+      for (int m : shell) s += std::fabs(c[m]);
+      best = std::max(best, s);
+    }
+    tail[j] = best;
+  });
+  return TBX_OK;
+}
+
+int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor, uint8_t *depth,
+                  int bc) {
+  if (!num_leaves || !anchor || !depth) return fail(TBX_EINVAL, "tbx_balance21: null argument");
+  int rc = check_bc(bc);
+  if (rc) return rc;
+  int64_t L = *num_leaves;
+  for (int round = 0; round < 64; ++round) {
+    std::vector<uint64_t> key(L);
+    for (int64_t j = 0; j < L; ++j)
+      key[j] = leaf_key(anchor[3 * j], anchor[3 * j + 1], anchor[3 * j + 2], depth[j]);
+    for (int64_t j = 1; j < L; ++j)
+      if (key[j] <= key[j - 1]) return fail(TBX_EORDER, "tbx_balance21: leaves not in Morton order");
+    std::unique_ptr<std::atomic<uint8_t>[]> split(new std::atomic<uint8_t>[L]);
+    for (int64_t j = 0; j < L; ++j) split[j].store(0, std::memory_order_relaxed);
+    std::atomic<int64_t> nsplit{0};
+    parallel_for(L, [&](int64_t j) {
+      const int dj = depth[j];
+      if (dj < 2) return;
+      const int64_t lim = (int64_t)1 << dj;
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            if (!dx && !dy && !dz) continue;
+            int64_t n[3] = {(int64_t)anchor[3 * j] + dx, (int64_t)anchor[3 * j + 1] + dy,
+                            (int64_t)anchor[3 * j + 2] + dz};
+            bool inside = true;
+            for (int a = 0; a < 3; ++a) {
+              if (n[a] < 0 || n[a] >= lim) {
+                if (bc == TBX_BC_PERIODIC)
+                  n[a] = (n[a] + lim) % lim;
+                else
+                  inside = false;
+              }
+            }
+            if (!inside) continue;
+            const uint64_t k = leaf_key((uint32_t)n[0], (uint32_t)n[1], (uint32_t)n[2], dj);
+            const int64_t jn = std::upper_bound(key.begin(), key.end(), k) - key.begin() - 1;
+            if (jn >= 0 && depth[jn] < dj - 1 && split[jn].exchange(1) == 0)
+              nsplit.fetch_add(1);
+          }
+    });
+    if (nsplit.load() == 0) break;
+    int64_t newL = 0;
+    for (int64_t j = 0; j < L; ++j) newL += split[j] ? 8 : 1;
+    if (newL > capacity) return fail(TBX_ENOMEM, "tbx_balance21: capacity %lld too small", (long long)capacity);
+    // expand back to front so the arrays can be rewritten in place
+    int64_t w = newL;
+    for (int64_t j = L - 1; j >= 0; --j) {
+      const uint32_t ax = anchor[3 * j], ay = anchor[3 * j + 1], az = anchor[3 * j + 2];
+      const uint8_t dj = depth[j];
+      if (split[j]) {
+        for (int c = 7; c >= 0; --c) {
+          --w;
+          anchor[3 * w + 0] = 2 * ax + (c & 1);
+          anchor[3 * w + 1] = 2 * ay + ((c >> 1) & 1);
+          anchor[3 * w + 2] = 2 * az + ((c >> 2) & 1);
+          depth[w] = dj + 1;
+        }
+      } else {
+        --w;
+        anchor[3 * w + 0] = ax;
+        anchor[3 * w + 1] = ay;
+        anchor[3 * w + 2] = az;
+        depth[w] = dj;
+      }
+    }
+    L = newL;
+  }
+  *num_leaves = L;
+  return TBX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// Internal interface between the host side of libtbx (tbx_host.cpp) and the
+// HIP kernel translation units.  Not installed; the public ABI is include/tbx.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tbx.h"
+
+namespace tbx {
+
+// Morton keys resolve TBX_MAX_DEPTH bits per axis: 3*21 = 63-bit keys,
+// z above y above x at every level (the child order PVFMM's MortonId uses).
+constexpr int kKeyBits = TBX_MAX_DEPTH;
+constexpr double kKeyScale = 2097152.0;  // 2^21
+
+enum Mode : int { kFma64 = 0, kExact64 = 1, kFp32 = 2 };
+
+// Device view of one tree.  All pointers are device memory owned by the
+// tree handle.
+struct TreeView {
+  const uint64_t *key;   // [L] Morton key of each leaf's anchor at 21 bits
+  const uint4 *info;     // [L] anchor x, y, z (units 2^-depth), depth
+  const int32_t *dir;    // [8^g + 1] leaf holding the first key of each
+                         // level-g cell; dir[8^g] = L - 1
+  const double *coeff;   // [L][dof][nc]
+  const float *coeff32;  // [L][dof][nc] FP32 mirror (TBX_FP32 paths)
+  int64_t L;
+  int32_t dof;
+  int32_t nc;
+  int32_t q;
+  int32_t dir_shift;     // 3 * (21 - g)
+};
+
+constexpr int kMaxSub = 16;  // RK substeps fused into one step launch
+
+// One IntegrateRK2 substep: stage factors of the time modulation.
+struct SubArgs {
+  double tau;   // (tfinal - tinit) / num_rk_step   (traj.inc:55)
+  double htau;  // 0.5 * tau                         (traj.inc:36)
+  double g1;    // g(t)            (1 when steady)
+  double g2;    // g(t + 0.5 tau)
+  int scaled;   // multiply by g1/g2
+  int bc;
+};
+
+struct StepArgs {
+  int64_t npts;       // L * P arrival points
+  int64_t nblocks;    // logical workgroups
+  int64_t per_xcd;    // logical workgroups per XCD
+  int32_t P;          // (q+1)^3
+  int32_t nsub;       // RK substeps
+  int32_t bc;
+  int32_t scaled;
+  double tau, htau;
+  double g[2 * kMaxSub];
+  double node[TBX_MAX_Q + 1];  // stretched nodes (cheb.h:53-58)
+};
+
+constexpr int kBlock = 256;
+
+// Launchers instantiated once per degree Q (tbx_kernels.hip, -DTBX_Q=Q).
+#define TBX_DECLARE_Q(Q)                                                      \
+  hipError_t launch_eval_q##Q(int mode, const TreeView &t, double *pts,       \
+                              int64_t n, int bc, int write_back, double *out, \
+                              hipStream_t s);                                 \
+  hipError_t launch_traj_q##Q(int mode, const TreeView &vel, double *x,       \
+                              int64_t n, const SubArgs &a, hipStream_t s);    \
+  hipError_t launch_step_q##Q(int mode, const TreeView &vel,                  \
+                              const TreeView &conc, const StepArgs &a,        \
+                              double *vals, double *dep, hipStream_t s);
+
+TBX_DECLARE_Q(1)
+TBX_DECLARE_Q(2)
+TBX_DECLARE_Q(3)
+TBX_DECLARE_Q(4)
+TBX_DECLARE_Q(5)
+TBX_DECLARE_Q(6)
+TBX_DECLARE_Q(7)
+TBX_DECLARE_Q(8)
+TBX_DECLARE_Q(9)
+TBX_DECLARE_Q(10)
+TBX_DECLARE_Q(11)
+TBX_DECLARE_Q(12)
+TBX_DECLARE_Q(13)
+TBX_DECLARE_Q(14)
+TBX_DECLARE_Q(15)
+TBX_DECLARE_Q(16)
+
+// Degree-independent launchers (tbx_common.hip).
+hipError_t launch_locate(const TreeView &t, const double *pts, int64_t n,
+                         int bc, int32_t *leaf, hipStream_t s);
+hipError_t launch_grid_points(const TreeView &t, const double *node, int d,
+                              double *out, hipStream_t s);
+hipError_t launch_to_f32(const double *in, float *out, int64_t n,
+                         hipStream_t s);
+
+}  // namespace tbx

@@ -1,0 +1,347 @@
+"""Linear octrees with Chebyshev leaves on the device, and the operations of
+the semi-Lagrangian hot path over the libtbx C ABI.
+
+Layout (PVFMM Cheb_Node, include/tbx.h): leaves in Morton order, each with an
+integer anchor in units of its own edge and a depth; coefficients
+[L][dof][Nc] as packed triangles i(z) -> j(y) -> k(x), i+j+k <= q.
+
+Device buffers are torch tensors on cuda; every call runs on torch's current
+stream.  Host-side builders (uniform / adaptive leaves, 2:1 balance, initial
+fit) are setup helpers in the same library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import BC_FREESPACE, BC_PERIODIC, MODES, WRITE_BACK_WRAP, check, lib, tfac
+
+
+def ncoef(q: int) -> int:
+    return (q + 1) * (q + 2) * (q + 3) // 6
+
+
+def _ptr(a) -> int:
+    if a is None:
+        return 0
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+_device_ready = False
+
+
+def _init_device() -> None:
+    """Initialise torch's HIP context before libtbx makes its first device
+    call: both share one HIP runtime, and torch's device counting does not
+    initialise it."""
+    global _device_ready
+    if not _device_ready:
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: libtbx device calls need an MI355X")
+        torch.cuda.init()
+        torch.empty(1, device="cuda")
+        _device_ready = True
+
+
+def _stream() -> int:
+    import torch
+
+    _init_device()
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _require_dev(t, dtype, name, shape_tail=None):
+    import torch
+
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a cuda tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if shape_tail is not None and tuple(t.shape[1:]) != tuple(shape_tail):
+        raise ValueError(f"{name} must have shape [n,{shape_tail}], got {tuple(t.shape)}")
+
+
+def _mode(mode: str) -> int:
+    try:
+        return MODES[mode]
+    except KeyError:
+        raise ValueError(f"mode must be one of {sorted(MODES)}") from None
+
+
+class Tree:
+    """One rank's octree on the GPU (the object NodeFieldFunctor points at,
+    tree_functor.h:793-815)."""
+
+    def __init__(self, anchor, depth, q: int, dof: int, coeff=None):
+        anchor = np.ascontiguousarray(anchor, dtype=np.uint32).reshape(-1, 3)
+        depth = np.ascontiguousarray(depth, dtype=np.uint8).reshape(-1)
+        if anchor.shape[0] != depth.shape[0]:
+            raise ValueError("anchor and depth lengths differ")
+        self.anchor, self.depth = anchor, depth
+        self.L = int(depth.shape[0])
+        self.q, self.dof = int(q), int(dof)
+        self.nc = ncoef(self.q)
+        self.P = (self.q + 1) ** 3
+        cf = None
+        if coeff is not None:
+            cf = np.ascontiguousarray(coeff, dtype=np.float64)
+            if cf.size != self.L * self.dof * self.nc:
+                raise ValueError("coefficient array has the wrong size")
+        self._validate_host()
+        _init_device()
+        desc = _lib._TreeDesc(self.L, _ptr(anchor), _ptr(depth), self.q, self.dof, _ptr(cf))
+        h = C.c_void_p()
+        check(lib().tbx_tree_create(C.byref(desc), C.byref(h)))
+        self._h = h
+
+    def _validate_host(self):
+        """Argument checks that need no device (the C ABI repeats them)."""
+        if self.L == 0:
+            raise _lib.TbxError(1, "tbx_tree_create: empty leaf list")
+        if not (1 <= self.q <= _lib.MAX_Q):
+            raise _lib.TbxError(7, f"tbx_tree_create: degree {self.q} outside 1..{_lib.MAX_Q}")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tbx_tree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        L, q, dof, md = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().tbx_tree_info(self._h, C.byref(L), C.byref(q), C.byref(dof), C.byref(md)))
+        return L.value, q.value, dof.value, md.value
+
+    def set_coeffs(self, coeff):
+        import torch
+
+        if isinstance(coeff, torch.Tensor):
+            _require_dev(coeff, torch.float64, "coeff")
+            if coeff.numel() != self.L * self.dof * self.nc:
+                raise ValueError("coefficient tensor has the wrong size")
+            check(lib().tbx_tree_set_coeffs(self._h, _ptr(coeff), 1, _stream()))
+        else:
+            cf = np.ascontiguousarray(coeff, dtype=np.float64)
+            if cf.size != self.L * self.dof * self.nc:
+                raise ValueError("coefficient array has the wrong size")
+            check(lib().tbx_tree_set_coeffs(self._h, _ptr(cf), 0, _stream()))
+
+    def get_coeffs(self) -> np.ndarray:
+        out = np.empty((self.L, self.dof, self.nc), dtype=np.float64)
+        check(lib().tbx_tree_get_coeffs(self._h, _ptr(out), 0, _stream()))
+        return out
+
+    def get_coeffs_device(self):
+        import torch
+
+        out = torch.empty((self.L, self.dof, self.nc), dtype=torch.float64, device="cuda")
+        check(lib().tbx_tree_get_coeffs(self._h, _ptr(out), 1, _stream()))
+        return out
+
+    def locate(self, pts, bc: int = BC_FREESPACE):
+        import torch
+
+        _require_dev(pts, torch.float64, "pts", (3,))
+        out = torch.empty(pts.shape[0], dtype=torch.int32, device=pts.device)
+        check(lib().tbx_locate(self._h, _ptr(pts), pts.shape[0], bc, _ptr(out), _stream()))
+        return out
+
+    def eval(self, pts, bc: int = BC_FREESPACE, mode: str = "fp64", write_back: bool = False):
+        """EvalTree: values [n][dof] at pts [n][3]."""
+        import torch
+
+        _require_dev(pts, torch.float64, "pts", (3,))
+        out = torch.empty((pts.shape[0], self.dof), dtype=torch.float64, device=pts.device)
+        flags = _mode(mode) | (WRITE_BACK_WRAP if write_back else 0)
+        check(lib().tbx_eval(self._h, _ptr(pts), pts.shape[0], bc, flags, _ptr(out), _stream()))
+        return out
+
+    def grid_points(self):
+        import torch
+
+        out = torch.empty((self.L * self.P, 3), dtype=torch.float64, device="cuda")
+        check(lib().tbx_collect_grid_points(self._h, _ptr(out), _stream()))
+        return out
+
+    def set_grid_values(self, vals):
+        """SetTreeGridValues with vals [L][dof][P] (dof-major per leaf)."""
+        import torch
+
+        _require_dev(vals, torch.float64, "vals")
+        if vals.numel() != self.L * self.dof * self.P:
+            raise ValueError("vals must hold L*dof*P values")
+        check(lib().tbx_set_grid_values(self._h, _ptr(vals), _stream()))
+
+
+def traj_rk2(vel: Tree, xinit, tinit: float, tfinal: float, num_rk_step: int,
+             bc: int = BC_FREESPACE, mode: str = "fp64", g=None, out=None):
+    """ComputeTrajRK2 (traj.inc:48-68) through vel's functor."""
+    import torch
+
+    _require_dev(xinit, torch.float64, "xinit", (3,))
+    xsol = torch.empty_like(xinit) if out is None else out
+    cb = tfac(g)
+    check(lib().tbx_traj_rk2(vel.handle, cb, None, _ptr(xinit), xinit.shape[0], tinit, tfinal,
+                             num_rk_step, bc, _mode(mode), _ptr(xsol), _stream()))
+    return xsol
+
+
+def semilag_rk2(vel: Tree, conc: Tree, pts, timestep: int, dt: float, num_rk_step: int = 1,
+                bc: int = BC_FREESPACE, mode: str = "fp64", g=None):
+    """SolveSemilagRK2 (semilag.inc:26-45): conc at the departure points."""
+    import torch
+
+    _require_dev(pts, torch.float64, "pts", (3,))
+    vals = torch.empty((pts.shape[0], conc.dof), dtype=torch.float64, device=pts.device)
+    cb = tfac(g)
+    check(lib().tbx_semilag_rk2(vel.handle, cb, None, conc.handle, _ptr(pts), pts.shape[0],
+                                timestep, dt, num_rk_step, bc, _mode(mode), _ptr(vals),
+                                _stream()))
+    return vals
+
+
+def semilag_insitu(vel: Tree, conc: Tree, timestep: int, dt: float, num_rk_step: int = 1,
+                   bc: int = BC_FREESPACE, mode: str = "fp64", g=None, dep=None):
+    """SolveSemilagInSitu (tree_semilag.h:92-135): one fused step, conc refit
+    in place.  dep (optional cuda tensor [L*P][3]) receives departure points."""
+    cb = tfac(g)
+    check(lib().tbx_semilag_insitu(vel.handle, cb, None, conc.handle, timestep, dt, num_rk_step,
+                                   bc, _mode(mode), _ptr(dep), _stream()))
+
+
+# ---------------------------------------------------------------------------
+# Host-side setup helpers
+# ---------------------------------------------------------------------------
+
+def new_nodes(q: int) -> np.ndarray:
+    x = np.empty(q + 1, dtype=np.float64)
+    check(lib().tbx_new_nodes(q, _ptr(x)))
+    return x
+
+
+def pt2coeff_matrix(q: int) -> np.ndarray:
+    M = np.empty(((q + 1) ** 3, ncoef(q)), dtype=np.float64)
+    check(lib().tbx_pt2coeff_matrix(q, _ptr(M)))
+    return M
+
+
+def uniform_leaves(depth: int):
+    L = 8 ** depth
+    anchor = np.empty((L, 3), dtype=np.uint32)
+    dep = np.empty(L, dtype=np.uint8)
+    check(lib().tbx_uniform_leaves(depth, _ptr(anchor), _ptr(dep)))
+    return anchor, dep
+
+
+def first_kind_points(anchor, depth, q: int) -> np.ndarray:
+    anchor = np.ascontiguousarray(anchor, dtype=np.uint32)
+    depth = np.ascontiguousarray(depth, dtype=np.uint8)
+    L = depth.shape[0]
+    out = np.empty((L, (q + 1) ** 3, 3), dtype=np.float64)
+    check(lib().tbx_first_kind_points(L, _ptr(anchor), _ptr(depth), q, _ptr(out)))
+    return out
+
+
+def fit_leaves(vals, q: int, dof: int) -> np.ndarray:
+    """vals [L][P][dof] at first_kind_points -> coeff [L][dof][Nc]."""
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    P = (q + 1) ** 3
+    L = vals.size // (P * dof)
+    out = np.empty((L, dof, ncoef(q)), dtype=np.float64)
+    check(lib().tbx_fit_leaves(L, q, dof, _ptr(vals), _ptr(out)))
+    return out
+
+
+def tail_norm(coeff, q: int, dof: int) -> np.ndarray:
+    coeff = np.ascontiguousarray(coeff, dtype=np.float64)
+    L = coeff.size // (dof * ncoef(q))
+    out = np.empty(L, dtype=np.float64)
+    check(lib().tbx_tail_norm(L, q, dof, _ptr(coeff), _ptr(out)))
+    return out
+
+
+def balance21(anchor, depth, bc: int = BC_FREESPACE, capacity: int | None = None):
+    L = len(depth)
+    cap = capacity or max(8 * L, L + 4096)
+    a = np.zeros((cap, 3), dtype=np.uint32)
+    d = np.zeros(cap, dtype=np.uint8)
+    a[:L] = anchor
+    d[:L] = depth
+    n = C.c_int64(L)
+    check(lib().tbx_balance21(C.byref(n), cap, _ptr(a), _ptr(d), bc))
+    return a[: n.value].copy(), d[: n.value].copy()
+
+
+def fit_function(fn, anchor, depth, q: int, dof: int, batch: int = 4096) -> np.ndarray:
+    """Initial fit of fn (numpy: pts [n,3] -> values [n,dof]) on every leaf."""
+    L = len(depth)
+    out = np.empty((L, dof, ncoef(q)), dtype=np.float64)
+    P = (q + 1) ** 3
+    for a in range(0, L, batch):
+        b = min(L, a + batch)
+        pts = first_kind_points(anchor[a:b], depth[a:b], q)
+        vals = np.asarray(fn(pts.reshape(-1, 3)), dtype=np.float64).reshape(b - a, P, dof)
+        out[a:b] = fit_leaves(vals, q, dof)
+    return out
+
+
+def _split(anchor, depth, mask):
+    """Replace every masked leaf by its 8 children, keeping Morton order."""
+    reps = np.where(mask, 8, 1)
+    idx = np.repeat(np.arange(len(depth)), reps)
+    na = anchor[idx].astype(np.uint32)
+    nd = depth[idx].astype(np.uint8)
+    child = np.zeros(len(idx), dtype=np.int64)
+    starts = np.cumsum(reps) - reps
+    for c in range(8):
+        sel = starts[mask] + c
+        child[sel] = c
+    m = np.repeat(mask, reps)
+    na[m, 0] = 2 * na[m, 0] + (child[m] & 1)
+    na[m, 1] = 2 * na[m, 1] + ((child[m] >> 1) & 1)
+    na[m, 2] = 2 * na[m, 2] + ((child[m] >> 2) & 1)
+    nd[m] = nd[m] + 1
+    return na, nd
+
+
+def adaptive_leaves(fn, q: int, dof: int, tol: float, max_depth: int, min_depth: int = 0,
+                    bc: int = BC_FREESPACE, balance: bool = True, batch: int = 4096):
+    """Refine from a uniform min_depth tree while a leaf's top-shell
+    coefficient sum (tbx_tail_norm) exceeds tol, up to max_depth, then
+    2:1-balance (the RefineTree + Balance21 roles, tree_utils.h:117-123)."""
+    anchor, depth = uniform_leaves(min_depth)
+    fresh = np.arange(len(depth))  # leaves not yet tested
+    while True:
+        cand = fresh[depth[fresh] < max_depth]
+        if cand.size == 0:
+            break
+        mask = np.zeros(len(depth), dtype=bool)
+        for a in range(0, cand.size, batch):
+            sel = cand[a:a + batch]
+            cf = fit_function(fn, anchor[sel], depth[sel], q, dof, batch=batch)
+            mask[sel] = tail_norm(cf, q, dof) > tol
+        if not mask.any():
+            break
+        reps = np.where(mask, 8, 1)
+        anchor, depth = _split(anchor, depth, mask)
+        fresh = np.nonzero(np.repeat(mask, reps))[0]
+    if balance:
+        anchor, depth = balance21(anchor, depth, bc)
+    return anchor, depth

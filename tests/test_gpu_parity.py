@@ -1,0 +1,237 @@
+"""Parity of the HIP path (libtbx.so through its C ABI) with the FP64 CPU
+oracle on the same inputs.
+
+Bars: leaf ids bit-exact; TBX_EXACT values, departure points and grid
+points bit-identical; TBX_FP64 (fused multiply-add) within 1e-12 relative
+L-inf; TBX_FP32 contraction within 2e-6 relative L-inf of the coefficient
+scale (single-precision rounding of ~Nc terms).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import edge_points, random_tree
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-6
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def dev(torch, a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def pair(tb, orc, anchor, depth, q, dof, coeff):
+    return tb.Tree(anchor, depth, q, dof, coeff), orc.Tree(anchor, depth, q, dof, coeff)
+
+
+def rel(a, b):
+    b = np.asarray(b)
+    return np.abs(np.asarray(a) - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_locate_bit_exact(torch, tb, orc, seed, bc):
+    anchor, depth = random_tree(seed, max_depth=6, p_split=0.3)
+    g, o = pair(tb, orc, anchor, depth, 2, 1, None)
+    rng = np.random.default_rng(seed)
+    pts = rng.random((40000, 3)) * (1.3 if bc else 1.0) - (0.15 if bc else 0.0)
+    j = rng.integers(0, len(depth), 5000)
+    h = 0.5 ** depth[j].astype(np.float64)
+    corner = anchor[j] * h[:, None]
+    pts = np.concatenate([pts, edge_points(), corner,
+                          corner + h[:, None] * rng.integers(0, 2, (5000, 3))])
+    got = g.locate(dev(torch, pts), bc).cpu().numpy()
+    assert (got == orc.locate(o, pts, bc)).all()
+
+
+def test_locate_full_size_property(torch, tb):
+    # 2^24 points on the 262144-leaf uniform tree: every point lies in its
+    # leaf's half-open box (size-independent)
+    anchor, depth = tb.uniform_leaves(6)
+    g = tb.Tree(anchor, depth, 1, 1)
+    torch.manual_seed(0)
+    pts = torch.rand((1 << 24, 3), dtype=torch.float64, device="cuda")
+    leaf = g.locate(pts, 0).long()
+    lo = dev(torch, anchor, torch.float64)[leaf] / 64.0
+    assert bool(((pts >= lo) & (pts < lo + 1.0 / 64)).all())
+
+
+@pytest.mark.parametrize("q,dof,bc", [(1, 1, 0), (4, 3, 0), (6, 1, 1), (8, 3, 1), (14, 1, 0),
+                                      (16, 2, 0)])
+def test_eval_modes_against_oracle(torch, tb, orc, q, dof, bc):
+    anchor, depth = random_tree(q, max_depth=4)
+    rng = np.random.default_rng(q)
+    coeff = rng.standard_normal((len(depth), dof, tb.ncoef(q))) / (1 + np.arange(tb.ncoef(q)))
+    g, o = pair(tb, orc, anchor, depth, q, dof, coeff)
+    pts = rng.random((20000, 3)) * 1.1 - 0.05
+    pts = np.concatenate([pts, edge_points()])
+    ref, wrapped = orc.eval_tree(o, pts, bc)
+    exact = g.eval(dev(torch, pts), bc, "exact").cpu().numpy()
+    assert np.array_equal(exact, ref), rel(exact, ref)
+    fma = g.eval(dev(torch, pts), bc, "fp64").cpu().numpy()
+    assert rel(fma, ref) <= FP64_TOL
+    f32 = g.eval(dev(torch, pts), bc, "fp32").cpu().numpy()
+    scale = np.abs(coeff).sum(axis=2).max()
+    assert np.abs(f32 - ref).max() <= FP32_TOL * scale
+    # EvalTree's in-place wrap, on request
+    p = dev(torch, pts)
+    g.eval(p, bc, "exact", write_back=True)
+    assert np.array_equal(p.cpu().numpy(), wrapped if bc else pts)
+
+
+def test_eval_every_degree_bitwise(torch, tb, orc):
+    anchor, depth = random_tree(99, max_depth=3)
+    rng = np.random.default_rng(5)
+    pts = rng.random((3000, 3))
+    for q in range(1, 17):
+        coeff = rng.standard_normal((len(depth), 3, tb.ncoef(q)))
+        g, o = pair(tb, orc, anchor, depth, q, 3, coeff)
+        ref, _ = orc.eval_tree(o, pts, 0)
+        assert np.array_equal(g.eval(dev(torch, pts), 0, "exact").cpu().numpy(), ref), q
+
+
+def test_eval_empty_and_errors(torch, tb):
+    anchor, depth = tb.uniform_leaves(1)
+    g = tb.Tree(anchor, depth, 3, 1)
+    out = g.eval(torch.empty((0, 3), dtype=torch.float64, device="cuda"))
+    assert out.shape == (0, 1)
+    with pytest.raises(tb.TbxError):
+        g.eval(torch.zeros((4, 3), dtype=torch.float64, device="cuda"), bc=7)
+
+
+def rotation_trees(tb, orc, depth, q, omega):
+    from tbslas_amd import fields
+
+    anchor, dep = tb.uniform_leaves(depth)
+    cf = tb.fit_function(lambda p: fields.rotation(p, omega), anchor, dep, q, 3)
+    return pair(tb, orc, anchor, dep, q, 3, cf)
+
+
+def tg_trees(tb, orc, anchor, dep, q):
+    from tbslas_amd import fields
+
+    cf = tb.fit_function(fields.taylor_green_2d3, anchor, dep, q, 3)
+    return pair(tb, orc, anchor, dep, q, 3, cf)
+
+
+@pytest.mark.parametrize("nrk,bc,timed", [(1, 0, False), (3, 1, True), (2, 0, True)])
+def test_traj_bit_identical(torch, tb, orc, nrk, bc, timed):
+    anchor, depth = random_tree(7, max_depth=4)
+    g, o = tg_trees(tb, orc, anchor, depth, 6)
+    rng = np.random.default_rng(nrk)
+    x = rng.random((30000, 3))
+    fac = (lambda t: math.cos(math.pi * t)) if timed else None
+    ref = orc.traj_rk2(o, x, 0.7, 0.6, nrk, bc, g=fac)
+    got = tb.traj_rk2(g, dev(torch, x), 0.7, 0.6, nrk, bc, "exact", g=fac).cpu().numpy()
+    assert np.array_equal(got, ref)
+    got = tb.traj_rk2(g, dev(torch, x), 0.7, 0.6, nrk, bc, "fp64", g=fac).cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-13
+    # in place, as ComputeTrajRK2 runs (xsol aliases xinit)
+    xi = dev(torch, x)
+    tb.traj_rk2(g, xi, 0.7, 0.6, nrk, bc, "exact", g=fac, out=xi)
+    assert np.array_equal(xi.cpu().numpy(), ref)
+
+
+def test_semilag_rk2_against_oracle(torch, tb, orc):
+    from tbslas_amd import fields
+
+    gv, ov = rotation_trees(tb, orc, 3, 6, 2 * math.pi)
+    anchor, dep = tb.uniform_leaves(3)
+    cf = tb.fit_function(lambda p: fields.gaussian(p, (0.7, 0.5, 0.5), 0.06), anchor, dep, 6, 1)
+    gc, oc = pair(tb, orc, anchor, dep, 6, 1, cf)
+    x = np.random.default_rng(1).random((20000, 3))
+    ref = orc.semilag_rk2(ov, oc, x, 3, 2 * math.pi / 200, 1, 0)
+    got = tb.semilag_rk2(gv, gc, dev(torch, x), 3, 2 * math.pi / 200, 1, 0, "exact").cpu().numpy()
+    assert np.array_equal(got, ref)
+
+
+def test_grid_points_and_refit(torch, tb, orc):
+    anchor, depth = random_tree(12, max_depth=4)
+    q, dof = 7, 2
+    g, o = pair(tb, orc, anchor, depth, q, dof, None)
+    gp = g.grid_points().cpu().numpy()
+    assert np.array_equal(gp, orc.collect_grid_points(o))
+    vals = np.random.default_rng(3).standard_normal((len(depth), dof, (q + 1) ** 3))
+    g.set_grid_values(dev(torch, vals))
+    orc.set_grid_values(o, orc.pt2coeff(q), vals)
+    assert rel(g.get_coeffs(), o.coeff) <= 1e-13
+
+
+@pytest.mark.parametrize("mode,bc,vfield", [("exact", 0, "tg"), ("fp64", 1, "tg"),
+                                            ("fp32", 0, "rot")])
+def test_insitu_steps_against_oracle(torch, tb, orc, mode, bc, vfield):
+    # FP32 uses a velocity the tree reproduces exactly (rigid rotation): with
+    # a velocity fit that jumps across leaf faces (Taylor-Green on coarse
+    # leaves, whose stagnation planes are leaf faces) a 1e-9 FP32 change of a
+    # midpoint on a face can pick the neighbouring leaf and move the
+    # departure point by dt times that jump.
+    from tbslas_amd import fields
+
+    q = 6
+    anchor, depth = random_tree(21, max_depth=3)
+    if vfield == "tg":
+        gv, ov = tg_trees(tb, orc, anchor, depth, q)
+    else:
+        vcf = tb.fit_function(lambda p: fields.rotation(p, 2 * math.pi), anchor, depth, q, 3)
+        gv, ov = pair(tb, orc, anchor, depth, q, 3, vcf)
+    cf = tb.fit_function(lambda p: fields.gaussian(p, (0.6, 0.45, 0.5), 0.1), anchor, depth, q, 1)
+    gc, oc = pair(tb, orc, anchor, depth, q, 1, cf)
+    M = orc.pt2coeff(q)
+    fac = lambda t: math.cos(math.pi * t)
+    dt = 0.02
+    dep = torch.empty((gc.L * gc.P, 3), dtype=torch.float64, device="cuda")
+    for step in range(1, 4):
+        # departure points of the oracle's step at the same state
+        pts = orc.collect_grid_points(oc)
+        ref_dep = orc.traj_rk2(ov, pts, step * dt, step * dt - dt, 2, bc, g=fac)
+        if bc == 1:
+            ref_dep = np.where(ref_dep < 0, ref_dep + 1, ref_dep)
+            ref_dep = np.where(ref_dep >= 1, ref_dep - 1, ref_dep)
+        orc.semilag_insitu(ov, oc, M, step, dt, 2, bc, g=fac)
+        tb.semilag_insitu(gv, gc, step, dt, 2, bc, mode, g=fac, dep=dep)
+        got_dep = dep.cpu().numpy()
+        if mode == "exact":
+            assert np.array_equal(got_dep, ref_dep)
+        elif mode == "fp64":
+            assert np.abs(got_dep - ref_dep).max() <= 1e-14
+        else:
+            assert np.abs(got_dep - ref_dep).max() <= 1e-8
+        err = np.abs(gc.get_coeffs() - oc.coeff).max() / np.abs(oc.coeff).max()
+        assert err <= (FP64_TOL if mode != "fp32" else 1e-5), (step, err)
+
+
+def test_c1_full_size_step(torch, tb, orc):
+    # the CPU-runnable workload at full size: uniform depth 4 (4096 leaves),
+    # q = 8, v = 2 pi (0.5 - y, x - 0.5, 0), Gaussian blob sigma 0.06 at
+    # (0.7, 0.5, 0.5), dt = 2 pi / 200; one step of 2,985,984 departure points
+    from tbslas_amd import fields
+
+    q, dt = 8, 2 * math.pi / 200
+    gv, ov = rotation_trees(tb, orc, 4, q, 2 * math.pi)
+    anchor, depth = tb.uniform_leaves(4)
+    cf = tb.fit_function(lambda p: fields.gaussian(p, (0.7, 0.5, 0.5), 0.06), anchor, depth, q, 1)
+    gc, oc = pair(tb, orc, anchor, depth, q, 1, cf)
+    pts = orc.collect_grid_points(oc)
+    ref_dep = orc.traj_rk2(ov, pts, dt, 0.0, 1, 0)
+    orc.semilag_insitu(ov, oc, orc.pt2coeff(q), 1, dt, 1, 0)
+    dep = torch.empty((gc.L * gc.P, 3), dtype=torch.float64, device="cuda")
+    tb.semilag_insitu(gv, gc, 1, dt, 1, 0, "exact", dep=dep)
+    assert np.array_equal(dep.cpu().numpy(), ref_dep)
+    leaf_ref = orc.locate(oc, ref_dep, 0)
+    assert (gc.locate(dep, 0).cpu().numpy() == leaf_ref).all()
+    assert rel(gc.get_coeffs(), oc.coeff) <= FP64_TOL
