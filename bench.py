@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark of the semi-Lagrangian step on one MI355X.
+
+Workload (adaptive Chebyshev octree, q = 14): a 2:1-balanced octree seeded
+at uniform depth 5 and refined (up to depth 7) where the top-shell Chebyshev
+tail of a Gaussian blob (sigma 0.04, centre (0.7, 0.5, 0.5)) exceeds 1e-13;
+36,324 leaves, 122,593,500 departure points per step.  Velocity: the
+Taylor-Green field (sin 2pi x cos 2pi y cos 2pi z, -cos 2pi x sin 2pi y
+cos 2pi z, 0) times cos(pi t), sampled on the same tree; periodic; dt = 1e-3;
+one RK2 substep.  A step is SolveSemilagInSitu: arrival points -> RK2 through
+the velocity tree -> concentration at the departure points -> refit.
+
+Reported: whole-job departure-point interpolations/s (3 per point per step:
+two velocity evaluations and one concentration evaluation) of the FP32
+contraction path (headline `value`) and of the plain FP64 path, the L-inf
+distance between them, the roofline of the dominant kernel measured with HIP
+events on the launch stream, and the FP64 CPU oracle timed on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "departure-pt interps/s on 1 GPU: batched FP32 vs plain FP64 at L∞≤1e-6"
+Q = 14
+SEED_DEPTH, MAX_DEPTH, TOL = 5, 7, 1e-13
+SIGMA, CENTRE = 0.04, (0.7, 0.5, 0.5)
+DT = 1e-3
+PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X published dense
+HBM_PEAK = 8000.0  # GB/s
+
+
+def flops_per_point(q: int) -> int:
+    """Algorithmic flops of one departure point in the step kernel: two
+    velocity evaluations (dof 3), one concentration evaluation, RK2 updates
+    (tree_functor.h:389-394 counts 3*d*3 + 2*Nc*dof per evaluation;
+    traj.inc:44 counts 18 per point per substep)."""
+    d = q + 1
+    nc = d * (d + 1) * (d + 2) // 6
+    return 2 * (9 * d + 2 * nc * 3) + (9 * d + 2 * nc * 1) + 18
+
+
+def build_workload():
+    import tbslas_amd as tb
+    from tbslas_amd import fields
+
+    fn = lambda p: fields.gaussian(p, CENTRE, SIGMA)
+    anchor, depth = tb.adaptive_leaves(fn, Q, 1, TOL, max_depth=MAX_DEPTH, min_depth=SEED_DEPTH,
+                                       bc=tb.BC_PERIODIC)
+    ccf = tb.fit_function(fn, anchor, depth, Q, 1)
+    vcf = tb.fit_function(fields.taylor_green_2d3, anchor, depth, Q, 3)
+    return anchor, depth, ccf, vcf
+
+
+def tfac(t: float) -> float:
+    return math.cos(math.pi * t)
+
+
+def run_mode(mode, vel, anchor, depth, ccf, steps, warmup):
+    import torch
+
+    import tbslas_amd as tb
+
+    conc = tb.Tree(anchor, depth, Q, 1, ccf)
+    vals = torch.empty((conc.L, 1, conc.P), dtype=torch.float64, device="cuda")
+    for s in range(1, warmup + 1):
+        tb.semilag_values(vel, conc, s, DT, 1, tb.BC_PERIODIC, mode, g=tfac, vals=vals)
+        conc.set_grid_values(vals)
+    conc.set_coeffs(ccf)  # timed run starts from the initial state
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s in range(1, steps + 1):
+        e = ev[s - 1]
+        e[0].record()
+        tb.semilag_values(vel, conc, s, DT, 1, tb.BC_PERIODIC, mode, g=tfac, vals=vals)
+        e[1].record()
+        conc.set_grid_values(vals)
+        e[2].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    k_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    r_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    return conc, wall, k_ms, r_ms
+
+
+def cpu_baseline(anchor, depth, ccf, vcf, n_leaves: int):
+    """FP64 oracle (the reference algorithm restated, OpenMP) on the arrival
+    points of n_leaves random leaves: SolveSemilagRK2 plus their refit."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    rng = np.random.default_rng(0)
+    sel = np.sort(rng.choice(len(depth), n_leaves, replace=False))
+    ov = orc.Tree(anchor, depth, Q, 3, vcf)
+    oc = orc.Tree(anchor, depth, Q, 1, ccf)
+    sub = orc.Tree(anchor[sel], depth[sel], Q, 1, ccf[sel])
+    pts = orc.collect_grid_points(sub)
+    M = orc.pt2coeff(Q)
+    t0 = time.perf_counter()
+    vals = orc.semilag_rk2(ov, oc, pts, 1, DT, 1, orc.BC_PERIODIC, g=tfac)
+    orc.set_grid_values(sub, M, vals.reshape(n_leaves, 1, -1))
+    t = time.perf_counter() - t0
+    return 3 * len(pts) / t, len(pts), t
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary, if one exists."""
+    p = os.path.join(ROOT, "profiles", "step_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-leaves", type=int, default=int(os.environ.get("TBX_CPU_LEAVES", 8000)))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import tbslas_amd as tb
+
+    t_setup = time.perf_counter()
+    anchor, depth, ccf, vcf = build_workload()
+    L = len(depth)
+    P = (Q + 1) ** 3
+    N = L * P
+    vel = tb.Tree(anchor, depth, Q, 3, vcf)
+    print(f"# workload: {L} leaves, {N} departure points/step, setup "
+          f"{time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+
+    res = {}
+    trees = {}
+    for mode in ("fp64", "fp32"):
+        conc, wall, k_ms, r_ms = run_mode(mode, vel, anchor, depth, ccf, args.steps, args.warmup)
+        trees[mode] = conc
+        ms = 1e3 * wall / args.steps
+        fl = flops_per_point(Q) * N
+        res[mode] = {
+            "value": 3 * N * args.steps / wall,
+            "ms_per_step": ms,
+            "departure_points_per_s": N * args.steps / wall,
+            "steps_per_s": args.steps / wall,
+            "step_kernel_ms": k_ms,
+            "refit_ms": r_ms,
+            "step_kernel_tflops": fl / (k_ms * 1e-3) / 1e12,
+        }
+        print(f"# {mode}: {ms:.2f} ms/step (step kernel {k_ms:.2f} ms, refit {r_ms:.2f} ms)",
+              file=sys.stderr, flush=True)
+
+    # L-inf between the two paths after the timed steps, at every grid point
+    g = trees["fp64"].grid_points()
+    a = trees["fp64"].eval(g.clone(), tb.BC_PERIODIC, "fp64")
+    b = trees["fp32"].eval(g, tb.BC_PERIODIC, "fp64")
+    linf = float((a - b).abs().max() / a.abs().max())
+    del g, a, b
+
+    def roof(mode):
+        k_s = res[mode]["step_kernel_ms"] * 1e-3
+        ach = flops_per_point(Q) * N / k_s / 1e12
+        kname = "k_step<kFp32>" if mode == "fp32" else "k_step<kFma64>"
+        return {"bound": "mfma", "achieved": ach, "peak": PEAK[mode], "unit": "TFLOP/s",
+                "frac": ach / PEAK[mode], "traffic": traffic_from_profiles(kname),
+                "kernel": kname, "flops_per_point": flops_per_point(Q),
+                "points_per_launch": N}
+
+    out = {
+        "metric": METRIC,
+        "value": res["fp32"]["value"],
+        "unit": "interps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["fp32"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "adaptive octree q=14 (seed depth 5, Gaussian sigma 0.04 tail > 1e-13, "
+                        "max depth 7, 2:1 balanced), Taylor-Green velocity x cos(pi t), "
+                        "periodic, dt=1e-3, 1 RK2 substep",
+            "leaves": L, "q": Q, "departure_points_per_step": N, "interps_per_step": 3 * N,
+            "dt": DT, "bc": "periodic",
+        },
+        "fp64": res["fp64"],
+        "fp32": res["fp32"],
+        "fp32_over_fp64": res["fp32"]["value"] / res["fp64"]["value"],
+        "linf_rel_fp32_vs_fp64": linf,
+        "roofline": roof("fp32"),
+        "roofline_fp64": roof("fp64"),
+    }
+    if not args.no_cpu:
+        v, npts, t = cpu_baseline(anchor, depth, ccf, vcf, args.cpu_leaves)
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {
+            "value": v, "unit": "interps/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle SolveSemilagRK2 + refit on the {npts} arrival points of "
+                      f"{args.cpu_leaves} random leaves of the same tree ({t:.1f} s)",
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

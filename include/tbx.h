@@ -151,6 +151,17 @@ int tbx_semilag_insitu(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
                        int num_rk_step, int bc, uint32_t flags, double *dep,
                        tbx_stream s);
 
+/* ---- the first two stages of SolveSemilagInSitu without the refit:
+ *      arrival points of conc's leaves (tree_semilag.h:105-115) through
+ *      SolveSemilagRK2 (tree_semilag.h:124-127).  vals receives [L][dof][P]
+ *      (the layout tbx_set_grid_values takes); dep, if not NULL, the
+ *      departure points [L*P][3].  tbx_semilag_insitu = this call followed
+ *      by tbx_set_grid_values(conc, vals). -------------------------------- */
+int tbx_semilag_values(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
+                       const tbx_tree *conc, int timestep, double dt,
+                       int num_rk_step, int bc, uint32_t flags, double *vals,
+                       double *dep, tbx_stream s);
+
 /* ---- host helpers (setup; no device work) ----------------------------- */
 /* new_nodes (cheb.h:41-58), x[q+1]. */
 int tbx_new_nodes(int q, double *x);

@@ -77,6 +77,20 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     return LIB
 
 
+def build_drivers() -> list[str]:
+    """C++ programs that use the engine through include/tbx/tbslas.hpp."""
+    out = []
+    cdir = os.path.join(ROOT, "tests", "cpp")
+    for name in ("advection_driver",):
+        exe = os.path.join(cdir, name)
+        _run([HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-ffp-contract=off",
+              f"-I{os.path.join(ROOT, 'include')}", os.path.join(cdir, name + ".cpp"),
+              f"-L{HERE}", "-ltbx", "-Wl,-rpath,$ORIGIN/../../tbslas_amd", "-Wl,-rpath,/opt/rocm/lib",
+              "-o", exe])
+        out.append(exe)
+    return out
+
+
 def build_oracle() -> str:
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
     odir = os.path.join(ROOT, "oracle")
@@ -86,4 +100,5 @@ def build_oracle() -> str:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_drivers())
     print(build_oracle())

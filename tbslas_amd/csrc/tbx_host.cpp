@@ -635,33 +635,33 @@ int tbx_set_grid_values(tbx_tree *t, const double *vals, tbx_stream s) {
   return refit(t, vals, (hipStream_t)s);
 }
 
-int tbx_semilag_insitu(const tbx_tree *velc, tbx_tfac_fn g, void *gctx, tbx_tree *conc,
-                       int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
-                       double *dep, tbx_stream s) {
-  tbx_tree *vel = const_cast<tbx_tree *>(velc);
-  if (!vel || !conc) return fail(TBX_EINVAL, "tbx_semilag_insitu: null tree");
-  if (vel->dof != 3) return fail(TBX_EINVAL, "tbx_semilag_insitu: velocity dof %d", vel->dof);
+namespace {
+
+// Fused arrival -> RK2 -> conc evaluation into vals [L][dof][P].
+int step_values(tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc, int timestep,
+                double dt, int num_rk_step, int bc, uint32_t flags, double *vals, double *dep,
+                hipStream_t st, const char *who) {
+  if (!vel || !conc) return fail(TBX_EINVAL, "%s: null tree", who);
+  if (!vals) return fail(TBX_EINVAL, "%s: null value buffer", who);
+  if (vel->dof != 3) return fail(TBX_EINVAL, "%s: velocity dof %d", who, vel->dof);
   if (vel->q != conc->q)
-    return fail(TBX_EUNSUPPORTED, "tbx_semilag_insitu: velocity q %d != concentration q %d",
-                vel->q, conc->q);
+    return fail(TBX_EUNSUPPORTED, "%s: velocity q %d != concentration q %d", who, vel->q,
+                conc->q);
   if (num_rk_step <= 0 || num_rk_step > kMaxSub)
-    return fail(TBX_EUNSUPPORTED, "tbx_semilag_insitu: num_rk_step %d outside 1..%d",
-                num_rk_step, kMaxSub);
+    return fail(TBX_EUNSUPPORTED, "%s: num_rk_step %d outside 1..%d", who, num_rk_step,
+                kMaxSub);
   int rc = check_bc(bc);
   if (rc) return rc;
   int mode;
   if ((rc = mode_of(flags, &mode))) return rc;
-  hipStream_t st = (hipStream_t)s;
   if ((rc = prepare_mode(vel, mode, st))) return rc;
   if ((rc = prepare_mode(conc, mode, st))) return rc;
   const int P = conc->P();
-  if (!conc->d_vals) HIP_TRY(hipMalloc(&conc->d_vals, sizeof(double) * (size_t)conc->L * conc->dof * P));
-
   StepArgs a{};
   a.npts = conc->L * P;
   a.nblocks = (a.npts + kBlock - 1) / kBlock;
   a.per_xcd = (a.nblocks + 7) / 8;
-  if (8 * a.per_xcd > INT32_MAX) return fail(TBX_EOVERFLOW, "tbx_semilag_insitu: grid too large");
+  if (8 * a.per_xcd > INT32_MAX) return fail(TBX_EOVERFLOW, "%s: grid too large", who);
   a.P = P;
   a.nsub = num_rk_step;
   a.bc = bc;
@@ -676,7 +676,30 @@ int tbx_semilag_insitu(const tbx_tree *velc, tbx_tfac_fn g, void *gctx, tbx_tree
     tcur = tcur + a.tau;
   }
   new_nodes_host(conc->q, a.node);
-  HIP_TRY(dispatch_step(conc->q, mode, vel->view(), conc->view(), a, conc->d_vals, dep, st));
+  HIP_TRY(dispatch_step(conc->q, mode, vel->view(), conc->view(), a, vals, dep, st));
+  return TBX_OK;
+}
+
+}  // namespace
+
+int tbx_semilag_values(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, const tbx_tree *conc,
+                       int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
+                       double *vals, double *dep, tbx_stream s) {
+  return step_values(const_cast<tbx_tree *>(vel), g, gctx, const_cast<tbx_tree *>(conc),
+                     timestep, dt, num_rk_step, bc, flags, vals, dep, (hipStream_t)s,
+                     "tbx_semilag_values");
+}
+
+int tbx_semilag_insitu(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc,
+                       int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
+                       double *dep, tbx_stream s) {
+  if (!conc) return fail(TBX_EINVAL, "tbx_semilag_insitu: null tree");
+  hipStream_t st = (hipStream_t)s;
+  if (!conc->d_vals)
+    HIP_TRY(hipMalloc(&conc->d_vals, sizeof(double) * (size_t)conc->L * conc->dof * conc->P()));
+  int rc = step_values(const_cast<tbx_tree *>(vel), g, gctx, conc, timestep, dt, num_rk_step,
+                       bc, flags, conc->d_vals, dep, st, "tbx_semilag_insitu");
+  if (rc) return rc;
   return refit(conc, conc->d_vals, st);
 }
 

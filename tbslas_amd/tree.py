@@ -226,6 +226,23 @@ def semilag_insitu(vel: Tree, conc: Tree, timestep: int, dt: float, num_rk_step:
                                    bc, _mode(mode), _ptr(dep), _stream()))
 
 
+def semilag_values(vel: Tree, conc: Tree, timestep: int, dt: float, num_rk_step: int = 1,
+                   bc: int = BC_FREESPACE, mode: str = "fp64", g=None, vals=None, dep=None):
+    """SolveSemilagInSitu without the refit: conc at the departure points of
+    its own grid points, returned as [L][dof][P] (the set_grid_values layout)."""
+    import torch
+
+    if vals is None:
+        vals = torch.empty((conc.L, conc.dof, conc.P), dtype=torch.float64, device="cuda")
+    _require_dev(vals, torch.float64, "vals")
+    if vals.numel() != conc.L * conc.dof * conc.P:
+        raise ValueError("vals must hold L*dof*P values")
+    cb = tfac(g)
+    check(lib().tbx_semilag_values(vel.handle, cb, None, conc.handle, timestep, dt, num_rk_step,
+                                   bc, _mode(mode), _ptr(vals), _ptr(dep), _stream()))
+    return vals
+
+
 # ---------------------------------------------------------------------------
 # Host-side setup helpers
 # ---------------------------------------------------------------------------
