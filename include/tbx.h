@@ -190,6 +190,13 @@ int tbx_tail_norm(int64_t num_leaves, int q, int dof, const double *coeff,
 int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
                   uint8_t *depth, int bc);
 
+/* ---- tuning knobs (process-wide) ---------------------------------------
+ * TBX_OPT_STEP_SHAPE: launch shape of the fused step kernel (0..3; see
+ * DESIGN notes in the source); the value is used by later step calls. */
+enum { TBX_OPT_STEP_SHAPE = 1 };
+int tbx_set_option(int option, int value);
+int tbx_get_option(int option, int *value);
+
 const char *tbx_last_error(void);
 const char *tbx_version(void);
 

@@ -20,6 +20,7 @@ FP32 = 0x2
 WRITE_BACK_WRAP = 0x10
 MAX_Q = 16
 MAX_DEPTH = 21
+OPT_STEP_SHAPE = 1
 
 MODES = {"fp64": FP64, "exact": EXACT, "fp32": FP32}
 
@@ -73,6 +74,8 @@ _SIGS = {
     "tbx_fit_leaves": (_I32, [_I64, _I32, _I32, _P, _P]),
     "tbx_tail_norm": (_I32, [_I64, _I32, _I32, _P, _P]),
     "tbx_balance21": (_I32, [C.POINTER(_I64), _I64, _P, _P, _I32]),
+    "tbx_set_option": (_I32, [_I32, _I32]),
+    "tbx_get_option": (_I32, [_I32, C.POINTER(_I32)]),
     "tbx_last_error": (C.c_char_p, []),
     "tbx_version": (C.c_char_p, []),
 }
@@ -113,3 +116,13 @@ def tfac(g):
     if g is None:
         return TFAC_FN()
     return TFAC_FN(lambda t, ctx: float(g(t)))
+
+
+def set_option(option: int, value: int) -> None:
+    check(lib().tbx_set_option(option, value))
+
+
+def get_option(option: int) -> int:
+    v = C.c_int()
+    check(lib().tbx_get_option(option, C.byref(v)))
+    return v.value

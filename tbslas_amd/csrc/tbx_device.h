@@ -28,17 +28,6 @@ struct ModeType<kFp32> {
   using T = float;
 };
 
-// Coefficient read path: address space 4 (constant) with a wave-uniform
-// address compiles to s_load_dwordx{2..16}; the data is read-only for the
-// whole launch.
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-
-template <typename T>
-__device__ __forceinline__ cptr<T> as_const(const T *p) {
-  return (cptr<T>)(p);
-}
-
 // acc + a*b.  kExact64 keeps the reference's separate multiply and add
 // (vec_eval: add_intrin(w, mul_intrin(px, c)), tree_functor.h:63-76);
 // the other modes fuse.
@@ -148,11 +137,76 @@ __device__ __forceinline__ void cheb_poly(double x, T *p) {
 // Triangular tensor contraction of one leaf and one component
 // (vec_eval, tree_functor.h:38-77): u = sum_i pz_i (sum_j py_j
 // (sum_k px_k c_ijk)), i+j+k < D, coefficients in packed order.
-// c must be wave-uniform.
+//
+// The coefficients are wave-uniform and stream through SGPRs: the packed
+// triangle is read in 64-byte chunks by s_load_dwordx16, double-buffered in
+// blocks of two chunks.  Block b+1 is requested before block b's FMAs run and
+// waited for after them.  Scalar loads may return out of order, so the only
+// safe wait is lgkmcnt(0); the wait names the next block's registers as
+// in/out operands, so no FMA that reads them can be scheduled above it.  The
+// loads are written out explicitly because the compiler, left to itself,
+// hoists hundreds of them to the top of the unrolled contraction and spills
+// the SGPRs into VGPR lanes.  A chunk may extend up to 60 bytes past the
+// coefficient array; the tree handle pads its coefficient buffers for that.
+template <typename T>
+struct Chunk;
+template <>
+struct Chunk<double> {
+  typedef double V __attribute__((ext_vector_type(8)));
+  static constexpr int N = 8;
+};
+template <>
+struct Chunk<float> {
+  typedef float V __attribute__((ext_vector_type(16)));
+  static constexpr int N = 16;
+};
+
+template <typename T, int OFF>
+__device__ __forceinline__ typename Chunk<T>::V sload_chunk(const T *base) {
+  typename Chunk<T>::V r;
+  asm volatile("s_load_dwordx16 %0, %1, %2" : "=s"(r) : "s"(base), "i"(OFF));
+  return r;
+}
+
+// Request both chunks of a block; the running sums pass through so that the
+// FMAs of the block being computed are not moved above the request.  The
+// destinations are early-clobber: the second load reads the base address
+// after the first one was issued, so the first destination must not share
+// registers with the base (a load returning early would overwrite it).
+template <typename T, typename V>
+__device__ __forceinline__ void sload_block(const T *base, V &a, V &b, T &w, T &v, T &u) {
+  constexpr int OFF1 = Chunk<T>::N * (int)sizeof(T);
+  asm volatile("s_load_dwordx16 %0, %5, 0\n\ts_load_dwordx16 %1, %5, %6"
+               : "=&s"(a), "=&s"(b), "+v"(w), "+v"(v), "+v"(u)
+               : "s"(base), "i"(OFF1));
+}
+
+template <typename V>
+__device__ __forceinline__ void wait_chunks(V &a, V &b) {
+  asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(a), "+s"(b));
+}
+
+// The same wait, also passing the running sums through: FMAs feeding them
+// must complete before it and FMAs after it cannot be hoisted above it, so
+// each block's FMAs stay between its own wait and the next one (otherwise
+// the compiler sinks all FMAs below all loads and every buffer stays live).
+template <typename V, typename T>
+__device__ __forceinline__ void wait_chunks(V &a, V &b, T &w, T &v, T &u) {
+  asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(a), "+s"(b), "+v"(w), "+v"(v), "+v"(u));
+}
+
 template <int D, int MODE, typename T>
-__device__ __forceinline__ T contract(const T *cg, const T *px, const T *py,
+__device__ __forceinline__ T contract(const T *c, const T *px, const T *py,
                                       const T *pz) {
-  const cptr<T> c = as_const(cg);
+  typedef typename Chunk<T>::V V;
+  constexpr int CN = Chunk<T>::N;  // coefficients per chunk
+  constexpr int BN = 2 * CN;       // coefficients per block
+  constexpr int NC = D * (D + 1) * (D + 2) / 6;
+  constexpr int NB = (NC + BN - 1) / BN;
+  constexpr int OFF1 = CN * (int)sizeof(T);
+  V cur0 = sload_chunk<T, 0>(c), cur1 = sload_chunk<T, OFF1>(c);
+  V nxt0 = cur0, nxt1 = cur1;
+  wait_chunks(cur0, cur1);
   T u = T(0);
   int idx = 0;
 #pragma unroll
@@ -163,7 +217,21 @@ __device__ __forceinline__ T contract(const T *cg, const T *px, const T *py,
       T w = T(0);
 #pragma unroll
       for (int k = 0; i + j + k < D; ++k) {
-        w = madd<MODE>(px[k], c[idx], w);
+        if (idx % BN == 0) {
+          if (idx > 0) {
+            // block idx/BN was requested one block ago
+            wait_chunks(nxt0, nxt1, w, v, u);
+            cur0 = nxt0;
+            cur1 = nxt1;
+          }
+          if (idx / BN + 1 < NB) {
+            // request the next block before this block's FMAs
+            sload_block(c + (idx / BN + 1) * BN, nxt0, nxt1, w, v, u);
+          }
+        }
+        const int r = idx % BN;
+        const T ck = r < CN ? cur0[r] : cur1[r - CN];
+        w = madd<MODE>(px[k], ck, w);
         ++idx;
       }
       v = madd<MODE>(py[j], w, v);
@@ -243,9 +311,15 @@ __device__ __forceinline__ void eval_vec3(const TreeView &t, double x,
   const T *base = coeff_base<MODE>(t);
   for_each_wave_leaf(b.leaf, [&](int lf) {
     const T *c = base + (size_t)lf * (3 * NC);
-    v0 = (double)contract<D, MODE>(c, b.px, b.py, b.pz);
-    v1 = (double)contract<D, MODE>(c + NC, b.px, b.py, b.pz);
-    v2 = (double)contract<D, MODE>(c + 2 * NC, b.px, b.py, b.pz);
+    // one component at a time: unrolled side by side, the three
+    // contractions' scalar loads exhaust the SGPRs and spill
+#pragma unroll 1
+    for (int l = 0; l < 3; ++l) {
+      const double u = (double)contract<D, MODE>(c + l * NC, b.px, b.py, b.pz);
+      v0 = l == 0 ? u : v0;
+      v1 = l == 1 ? u : v1;
+      v2 = l == 2 ? u : v2;
+    }
   });
 }
 

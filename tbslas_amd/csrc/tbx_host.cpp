@@ -27,6 +27,7 @@ using namespace tbx;
 namespace {
 
 thread_local std::string g_err;
+std::atomic<int> g_step_shape{0};
 
 int fail(int code, const char *fmt, ...) {
   char buf[512];
@@ -47,6 +48,11 @@ int fail(int code, const char *fmt, ...) {
   } while (0)
 
 inline int ncoef(int q) { return (q + 1) * (q + 2) * (q + 3) / 6; }
+
+// The contraction streams coefficients in 64-byte scalar chunks, two per
+// block, and may read up to 124 bytes past the last triangle; coefficient
+// buffers carry this much slack.
+constexpr size_t kCoeffPad = 256;
 
 unsigned host_threads() {
   unsigned h = std::thread::hardware_concurrency();
@@ -271,7 +277,7 @@ void free_tree(tbx_tree *t) {
 
 int ensure_f32(tbx_tree *t, hipStream_t s) {
   if (!t->d_coeff32) {
-    HIP_TRY(hipMalloc(&t->d_coeff32, sizeof(float) * (size_t)t->ncoeff_total()));
+    HIP_TRY(hipMalloc(&t->d_coeff32, sizeof(float) * (size_t)t->ncoeff_total() + kCoeffPad));
     t->f32_stale = true;
   }
   if (t->f32_stale) {
@@ -372,7 +378,7 @@ int ensure_refit(tbx_tree *t, hipStream_t s) {
     HIP_TRY(hipMemcpy(t->d_M, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice));
   }
   if (!t->d_coeff_next)
-    HIP_TRY(hipMalloc(&t->d_coeff_next, sizeof(double) * (size_t)t->ncoeff_total()));
+    HIP_TRY(hipMalloc(&t->d_coeff_next, sizeof(double) * (size_t)t->ncoeff_total() + kCoeffPad));
   if (!t->blas) {
     if (rocblas_create_handle(&t->blas) != rocblas_status_success)
       return fail(TBX_EHIP, "rocblas_create_handle failed");
@@ -495,7 +501,7 @@ int tbx_tree_create(const tbx_tree_desc *desc, tbx_tree **out) {
   if ((e = hipMalloc(&t->d_key, sizeof(uint64_t) * L)) != hipSuccess) return bad(e, "hipMalloc");
   if ((e = hipMalloc(&t->d_info, sizeof(uint4) * L)) != hipSuccess) return bad(e, "hipMalloc");
   if ((e = hipMalloc(&t->d_dir, sizeof(int32_t) * dir.size())) != hipSuccess) return bad(e, "hipMalloc");
-  if ((e = hipMalloc(&t->d_coeff, sizeof(double) * (size_t)t->ncoeff_total())) != hipSuccess)
+  if ((e = hipMalloc(&t->d_coeff, sizeof(double) * (size_t)t->ncoeff_total() + kCoeffPad)) != hipSuccess)
     return bad(e, "hipMalloc");
   if ((e = hipMemcpy(t->d_key, t->h_key.data(), sizeof(uint64_t) * L, hipMemcpyHostToDevice)) != hipSuccess)
     return bad(e, "hipMemcpy");
@@ -658,8 +664,10 @@ int step_values(tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc, int ti
   if ((rc = prepare_mode(conc, mode, st))) return rc;
   const int P = conc->P();
   StepArgs a{};
+  const int want = g_step_shape.load();
+  a.shape = step_shape_built(want, conc->q) ? want : 0;
   a.npts = conc->L * P;
-  a.nblocks = (a.npts + kBlock - 1) / kBlock;
+  a.nblocks = (a.npts + kStepShapeBlock[a.shape] - 1) / kStepShapeBlock[a.shape];
   a.per_xcd = (a.nblocks + 7) / 8;
   if (8 * a.per_xcd > INT32_MAX) return fail(TBX_EOVERFLOW, "%s: grid too large", who);
   a.P = P;
@@ -701,6 +709,26 @@ int tbx_semilag_insitu(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree 
                        bc, flags, conc->d_vals, dep, st, "tbx_semilag_insitu");
   if (rc) return rc;
   return refit(conc, conc->d_vals, st);
+}
+
+int tbx_set_option(int option, int value) {
+  if (option == TBX_OPT_STEP_SHAPE) {
+    if (value < 0 || value >= kStepShapes)
+      return fail(TBX_EINVAL, "tbx_set_option: step shape %d outside 0..%d", value,
+                  kStepShapes - 1);
+    g_step_shape = value;
+    return TBX_OK;
+  }
+  return fail(TBX_EINVAL, "tbx_set_option: unknown option %d", option);
+}
+
+int tbx_get_option(int option, int *value) {
+  if (!value) return fail(TBX_EINVAL, "tbx_get_option: null output");
+  if (option == TBX_OPT_STEP_SHAPE) {
+    *value = g_step_shape;
+    return TBX_OK;
+  }
+  return fail(TBX_EINVAL, "tbx_get_option: unknown option %d", option);
 }
 
 int tbx_new_nodes(int q, double *x) {

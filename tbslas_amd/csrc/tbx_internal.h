@@ -44,7 +44,15 @@ struct SubArgs {
   int bc;
 };
 
+// Launch shapes of the fused step kernel: {workgroup size, minimum waves
+// per SIMD}.  Shape 0 is built for every degree; the others for the degrees
+// the benchmark workloads use (8 and 14); other degrees fall back to 0.
+constexpr int kStepShapes = 4;
+constexpr int kStepShapeBlock[kStepShapes] = {256, 256, 1024, 512};
+inline bool step_shape_built(int shape, int q) { return shape == 0 || q == 8 || q == 14; }
+
 struct StepArgs {
+  int32_t shape;      // index into kStepShapeBlock
   int64_t npts;       // L * P arrival points
   int64_t nblocks;    // logical workgroups
   int64_t per_xcd;    // logical workgroups per XCD

@@ -59,16 +59,18 @@ __global__ __launch_bounds__(kBlock) void k_traj(TreeView vel,
 // (leaf, node) -> num_rk_step RK2 substeps through vel -> conc at the
 // departure point -> vals[leaf][l][node] for the refit.  Workgroups are
 // remapped so that each XCD runs a contiguous range of arrival points and a
-// leaf's coefficients are fetched into one XCD's L2.
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_step(TreeView vel, TreeView conc,
-                                                 StepArgs a,
-                                                 double *__restrict__ vals,
-                                                 double *__restrict__ dep) {
+// leaf's coefficients are fetched into one XCD's L2.  BLOCK and WPE (the
+// minimum waves per SIMD the register budget must allow) select the launch
+// shape: wide workgroups keep the waves resident on a CU inside one or two
+// leaves, so their scalar coefficient loads hit the same cache lines.
+template <int MODE, int BLOCK, int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_step(
+    TreeView vel, TreeView conc, StepArgs a, double *__restrict__ vals,
+    double *__restrict__ dep) {
   const uint32_t b = blockIdx.x;
   const int64_t lb = (int64_t)(b & 7u) * a.per_xcd + (int64_t)(b >> 3);
   if (lb >= a.nblocks) return;
-  const int64_t i = lb * kBlock + threadIdx.x;
+  const int64_t i = lb * BLOCK + threadIdx.x;
   if (i >= a.npts) return;
   const int64_t leaf = i / a.P;
   const int m = (int)(i - leaf * a.P);
@@ -154,28 +156,52 @@ hipError_t TBX_CAT(launch_traj_q, TBX_Q)(int mode, const TreeView &vel,
   return hipGetLastError();
 }
 
+namespace {
+
+template <int MODE, int BLOCK, int WPE>
+hipError_t launch_step_shape(const TreeView &vel, const TreeView &conc, const StepArgs &a,
+                             double *vals, double *dep, hipStream_t s) {
+  hipLaunchKernelGGL((k_step<MODE, BLOCK, WPE>), dim3((unsigned)(8 * a.per_xcd)), dim3(BLOCK), 0,
+                     s, vel, conc, a, vals, dep);
+  return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_step_mode(const TreeView &vel, const TreeView &conc, const StepArgs &a,
+                            double *vals, double *dep, hipStream_t s) {
+  // launch shapes of kStepShapes (tbx_internal.h); the extra shapes are
+  // built for the degrees of the benchmark workloads only
+  switch (a.shape) {
+#if TBX_Q == 8 || TBX_Q == 14
+    case 1:
+      return launch_step_shape<MODE, 256, 4>(vel, conc, a, vals, dep, s);
+    case 2:
+      return launch_step_shape<MODE, 1024, 4>(vel, conc, a, vals, dep, s);
+    case 3:
+      return launch_step_shape<MODE, 512, 4>(vel, conc, a, vals, dep, s);
+#endif
+    default:
+      return launch_step_shape<MODE, 256, 1>(vel, conc, a, vals, dep, s);
+  }
+}
+
+}  // namespace
+
 hipError_t TBX_CAT(launch_step_q, TBX_Q)(int mode, const TreeView &vel,
                                          const TreeView &conc,
                                          const StepArgs &a, double *vals,
                                          double *dep, hipStream_t s) {
   if (a.npts <= 0) return hipSuccess;
-  const dim3 g((unsigned)(8 * a.per_xcd)), blk(kBlock);
   switch (mode) {
     case kFma64:
-      hipLaunchKernelGGL(k_step<kFma64>, g, blk, 0, s, vel, conc, a, vals,
-                         dep);
-      break;
+      return launch_step_mode<kFma64>(vel, conc, a, vals, dep, s);
     case kExact64:
-      hipLaunchKernelGGL(k_step<kExact64>, g, blk, 0, s, vel, conc, a, vals,
-                         dep);
-      break;
+      return launch_step_mode<kExact64>(vel, conc, a, vals, dep, s);
     case kFp32:
-      hipLaunchKernelGGL(k_step<kFp32>, g, blk, 0, s, vel, conc, a, vals, dep);
-      break;
+      return launch_step_mode<kFp32>(vel, conc, a, vals, dep, s);
     default:
       return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 }  // namespace tbx
