@@ -82,11 +82,31 @@ typedef struct {
   const double *coeff;    /* host [L][dof][Nc], or NULL for zeros       */
 } tbx_tree_desc;
 
+
 /* Scalar time modulation g(t) of a steady velocity tree, v(x,t) = g(t) V(x);
  * NULL means steady.  Evaluated on the host once per RK stage.  It takes the
  * role of the time argument of the functor concept (traj.inc:33, 40), which
  * NodeFieldFunctor ignores (tree_functor.h:808-811). */
 typedef double (*tbx_tfac_fn)(double t, void *ctx);
+
+/* A velocity functor (dof 3 trees of equal degree):
+ *  TBX_VEL_TREE   NodeFieldFunctor(trees[0]) (tree_functor.h:793-815),
+ *                 scaled by g(t) when g is not NULL;
+ *  TBX_VEL_EXTRAP the extrapolating RK2 overloads (traj.inc:70-115):
+ *                 NodeFieldFunctor(trees[0]) for the first stage,
+ *                 FieldExtrapFunctor(trees[1], trees[0]) = 1.5 v(t^n) -
+ *                 0.5 v(t^n-1) for the second (tree_extrap_functor.h:47-78);
+ *                 trees[0] current, trees[1] previous;
+ *  TBX_VEL_SET    FieldSetFunctor(trees[0..3], times[0..3]): cubic Hermite
+ *                 in time across four trees (tree_set_functor.h:49-79). */
+enum { TBX_VEL_TREE = 0, TBX_VEL_EXTRAP = 1, TBX_VEL_SET = 2 };
+typedef struct {
+  int32_t kind;
+  const struct tbx_tree_s *trees[4];
+  double times[4];
+  tbx_tfac_fn g;
+  void *gctx;
+} tbx_velocity;
 
 /* ---- tree handle: replaces the pvfmm::MPI_Tree<Cheb_Node<double>> that
  *      NodeFieldFunctor points at (tree_functor.h:796, 814) -------------- */
@@ -161,6 +181,23 @@ int tbx_semilag_values(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
                        const tbx_tree *conc, int timestep, double dt,
                        int num_rk_step, int bc, uint32_t flags, double *vals,
                        double *dep, tbx_stream s);
+
+/* ---- the same entry points for any velocity functor ------------------- */
+/* The functor's value at pts at time t: a scaled tree, FieldExtrapFunctor's
+ * extrapolation, or FieldSetFunctor's Hermite value.  out [n][3]. */
+int tbx_velocity_eval(const tbx_velocity *vel, double *pts, int64_t n, double t,
+                      int bc, uint32_t flags, double *out, tbx_stream s);
+int tbx_traj_rk2_v(const tbx_velocity *vel, const double *xinit, int64_t n,
+                   double tinit, double tfinal, int num_rk_step, int bc,
+                   uint32_t flags, double *xsol, tbx_stream s);
+int tbx_semilag_values_v(const tbx_velocity *vel, const tbx_tree *conc,
+                         int timestep, double dt, int num_rk_step, int bc,
+                         uint32_t flags, double *vals, double *dep,
+                         tbx_stream s);
+/* SolveSemilagInSitu, both overloads (tree_semilag.h:92-181). */
+int tbx_semilag_insitu_v(const tbx_velocity *vel, tbx_tree *conc, int timestep,
+                         double dt, int num_rk_step, int bc, uint32_t flags,
+                         double *dep, tbx_stream s);
 
 /* ---- host helpers (setup; no device work) ----------------------------- */
 /* new_nodes (cheb.h:41-58), x[q+1]. */

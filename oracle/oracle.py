@@ -32,6 +32,18 @@ class OrcTree(C.Structure):
     ]
 
 
+class OrcVelocity(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("t", C.c_void_p * 4),
+        ("tt", C.c_double * 4),
+        ("g", TFAC_FN),
+        ("gctx", C.c_void_p),
+    ]
+
+
+VEL_TREE, VEL_EXTRAP, VEL_SET = 0, 1, 2
+
 _lib = None
 
 
@@ -63,6 +75,10 @@ def lib():
             "orc_cubic1d": (C.c_double, [C.c_double, P, P]),
             "orc_field_set_eval": (C.c_int, [P, P, P, C.c_int64, C.c_double, C.c_int, P]),
             "orc_field_extrap_eval": (C.c_int, [P, P, P, C.c_int64, C.c_int, P]),
+            "orc_traj_rk2_v": (C.c_int, [P, P, C.c_int64, C.c_double, C.c_double, C.c_int,
+                                         C.c_int, P]),
+            "orc_semilag_insitu_v": (C.c_int, [P, P, P, C.c_int, C.c_double, C.c_int,
+                                               C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -228,3 +244,32 @@ def field_extrap_eval(tp, tc, pts, bc=BC_FREESPACE):
     _check(lib().orc_field_extrap_eval(tp.ref, tc.ref, _p(p), len(p), bc, _p(out)),
            "field_extrap_eval")
     return out
+
+
+def velocity(kind, trees, times=None, g=None):
+    """A velocity functor: VEL_TREE (trees[0], optional g), VEL_EXTRAP
+    (trees[0] current, trees[1] previous) or VEL_SET (4 trees at times)."""
+    v = OrcVelocity()
+    v.kind = kind
+    for i, t in enumerate(trees):
+        v.t[i] = C.addressof(t.s)
+    if times is not None:
+        for i, x in enumerate(times):
+            v.tt[i] = x
+    cb = _tfac(g)
+    v.g = cb
+    v._keep = (trees, cb)
+    return v
+
+
+def traj_rk2_v(vel, xinit, tinit, tfinal, num_rk_step, bc=BC_FREESPACE):
+    x = np.ascontiguousarray(xinit, dtype=np.float64).reshape(-1, 3)
+    out = np.empty_like(x)
+    _check(lib().orc_traj_rk2_v(C.byref(vel), _p(x), len(x), tinit, tfinal, num_rk_step, bc,
+                                _p(out)), "traj_rk2_v")
+    return out
+
+
+def semilag_insitu_v(vel, conc, M, timestep, dt, num_rk_step=1, bc=BC_FREESPACE):
+    _check(lib().orc_semilag_insitu_v(C.byref(vel), conc.ref, _p(np.ascontiguousarray(M)),
+                                      timestep, dt, num_rk_step, bc), "semilag_insitu_v")

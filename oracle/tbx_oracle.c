@@ -848,3 +848,93 @@ int orc_field_extrap_eval(const orc_tree *tp, const orc_tree *tc, double *pts,
   free(vc);
   return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Velocity functors and the extrapolating RK2                               */
+/* ------------------------------------------------------------------------ */
+
+/* stage 1 / 2 of one IntegrateRK2 call: the 4-argument functor call with the
+ * stage time (traj.inc:33, 40), or field_fn / extrap_fn (traj.inc:80, 86) */
+static int vel_stage(const orc_velocity *v, int stage, double *pts, int64_t n,
+                     double t, int bc, double *out) {
+  switch (v->kind) {
+    case ORC_VEL_TREE:
+      return vel_eval(v->t[0], v->g, v->gctx, pts, n, t, bc, out);
+    case ORC_VEL_EXTRAP:
+      if (stage == 1) return orc_eval_tree(v->t[0], pts, n, bc, out);
+      return orc_field_extrap_eval(v->t[1], v->t[0], pts, n, bc, out);
+    case ORC_VEL_SET:
+      return orc_field_set_eval(v->t, v->tt, pts, n, t, bc, out);
+  }
+  return ORC_EINVAL;
+}
+
+int orc_traj_rk2_v(const orc_velocity *vel, const double *xinit, int64_t n,
+                   double tinit, double tfinal, int num_rk_step, int bc,
+                   double *xsol) {
+  if (!vel || n < 0 || num_rk_step <= 0) return ORC_EINVAL;
+  if (n == 0) return ORC_OK;
+  const int64_t m = 3 * n;
+  const double tau = (tfinal - tinit) / num_rk_step;
+  if (xsol != xinit) memcpy(xsol, xinit, sizeof(double) * (size_t)m);
+  double *xtmp = (double *)malloc(sizeof(double) * (size_t)m);
+  double *vtmp = (double *)malloc(sizeof(double) * (size_t)m);
+  if (!xtmp || !vtmp) {
+    free(xtmp);
+    free(vtmp);
+    return ORC_ENOMEM;
+  }
+  double tcur = tinit;
+  int rc = ORC_OK;
+  for (int s = 0; s < num_rk_step && !rc; s++) {
+    rc = vel_stage(vel, 1, xsol, n, tcur, bc, vtmp);
+    if (rc) break;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; i++) xtmp[i] = xsol[i] + 0.5 * tau * vtmp[i];
+    rc = vel_stage(vel, 2, xtmp, n, tcur + 0.5 * tau, bc, vtmp);
+    if (rc) break;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; i++) xsol[i] = xsol[i] + tau * vtmp[i];
+    tcur = tcur + tau;
+  }
+  free(xtmp);
+  free(vtmp);
+  return rc;
+}
+
+int orc_semilag_insitu_v(const orc_velocity *vel, orc_tree *conc,
+                         const double *M, int timestep, double dt,
+                         int num_rk_step, int bc) {
+  int rc = check_tree(conc);
+  if (rc) return rc;
+  const int d = conc->q + 1, P = d * d * d, dof = conc->dof;
+  const int64_t n = conc->num_leaves * P;
+  double *pts = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double *dep = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double *vals = (double *)malloc(sizeof(double) * dof * (size_t)n);
+  double *vt = (double *)malloc(sizeof(double) * dof * (size_t)n);
+  if (!pts || !dep || !vals || !vt) {
+    free(pts);
+    free(dep);
+    free(vals);
+    free(vt);
+    return ORC_ENOMEM;
+  }
+  rc = orc_collect_grid_points(conc, pts);
+  const double tinit = timestep * dt, tfinal = tinit - dt;
+  if (!rc) rc = orc_traj_rk2_v(vel, pts, n, tinit, tfinal, num_rk_step, bc, dep);
+  if (!rc) rc = orc_eval_tree(conc, dep, n, bc, vals);
+  if (!rc) {
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < conc->num_leaves; j++)
+      for (int l = 0; l < dof; l++)
+        for (int p = 0; p < P; p++)
+          vt[((size_t)j * dof + l) * P + p] = vals[((size_t)j * P + p) * dof + l];
+    rc = orc_set_grid_values(conc, M, vt);
+  }
+  free(pts);
+  free(dep);
+  free(vals);
+  free(vt);
+  return rc;
+}

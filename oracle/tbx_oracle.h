@@ -126,6 +126,33 @@ int orc_field_set_eval(const orc_tree *const *trees, const double *tt,
 int orc_field_extrap_eval(const orc_tree *tp, const orc_tree *tc, double *pts,
                           int64_t n, int bc, double *out);
 
+/* A velocity functor of the kinds the reference's trajectory templates take:
+ *  ORC_VEL_TREE   NodeFieldFunctor(t[0]) (tree_functor.h:793-815), optionally
+ *                 scaled by g(t);
+ *  ORC_VEL_EXTRAP the extrapolating overloads (traj.inc:70-115): field_fn =
+ *                 NodeFieldFunctor(t[0]) (current), extrap_fn =
+ *                 FieldExtrapFunctor(t[1] previous, t[0] current);
+ *  ORC_VEL_SET    FieldSetFunctor(t[0..3], tt[0..3]) (tree_set_functor.h). */
+#define ORC_VEL_TREE 0
+#define ORC_VEL_EXTRAP 1
+#define ORC_VEL_SET 2
+typedef struct {
+  int32_t kind;
+  const orc_tree *t[4];
+  double tt[4];
+  orc_tfac_fn g;
+  void *gctx;
+} orc_velocity;
+
+/* ComputeTrajRK2 with either overload (traj.inc:48-68, 97-115). */
+int orc_traj_rk2_v(const orc_velocity *vel, const double *xinit, int64_t n,
+                   double tinit, double tfinal, int num_rk_step, int bc,
+                   double *xsol);
+/* SolveSemilagInSitu with either overload (tree_semilag.h:92-181). */
+int orc_semilag_insitu_v(const orc_velocity *vel, orc_tree *conc,
+                         const double *M, int timestep, double dt,
+                         int num_rk_step, int bc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,6 +21,9 @@ WRITE_BACK_WRAP = 0x10
 MAX_Q = 16
 MAX_DEPTH = 21
 OPT_STEP_SHAPE = 1
+VEL_TREE = 0
+VEL_EXTRAP = 1
+VEL_SET = 2
 
 MODES = {"fp64": FP64, "exact": EXACT, "fp32": FP32}
 
@@ -41,6 +44,16 @@ class _TreeDesc(C.Structure):
         ("q", C.c_int32),
         ("dof", C.c_int32),
         ("coeff", C.c_void_p),
+    ]
+
+
+class _Velocity(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("trees", C.c_void_p * 4),
+        ("times", C.c_double * 4),
+        ("g", TFAC_FN),
+        ("gctx", C.c_void_p),
     ]
 
 
@@ -67,6 +80,13 @@ _SIGS = {
     "tbx_set_grid_values": (_I32, [_P, _P, _P]),
     "tbx_semilag_insitu": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P]),
     "tbx_semilag_values": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P, _P]),
+    "tbx_velocity_eval": (_I32, [C.POINTER(_Velocity), _P, _I64, _D, _I32, _U32, _P, _P]),
+    "tbx_traj_rk2_v": (_I32, [C.POINTER(_Velocity), _P, _I64, _D, _D, _I32, _I32, _U32, _P,
+                              _P]),
+    "tbx_semilag_values_v": (_I32, [C.POINTER(_Velocity), _P, _I32, _D, _I32, _I32, _U32, _P,
+                                    _P, _P]),
+    "tbx_semilag_insitu_v": (_I32, [C.POINTER(_Velocity), _P, _I32, _D, _I32, _I32, _U32, _P,
+                                    _P]),
     "tbx_new_nodes": (_I32, [_I32, _P]),
     "tbx_pt2coeff_matrix": (_I32, [_I32, _P]),
     "tbx_uniform_leaves": (_I32, [_I32, _P, _P]),

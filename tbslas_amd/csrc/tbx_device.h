@@ -139,60 +139,44 @@ __device__ __forceinline__ void cheb_poly(double x, T *p) {
 // (sum_k px_k c_ijk)), i+j+k < D, coefficients in packed order.
 //
 // The coefficients are wave-uniform and stream through SGPRs: the packed
-// triangle is read in 64-byte chunks by s_load_dwordx16, double-buffered in
-// blocks of two chunks.  Block b+1 is requested before block b's FMAs run and
-// waited for after them.  Scalar loads may return out of order, so the only
-// safe wait is lgkmcnt(0); the wait names the next block's registers as
-// in/out operands, so no FMA that reads them can be scheduled above it.  The
-// loads are written out explicitly because the compiler, left to itself,
-// hoists hundreds of them to the top of the unrolled contraction and spills
-// the SGPRs into VGPR lanes.  A chunk may extend up to 60 bytes past the
-// coefficient array; the tree handle pads its coefficient buffers for that.
+// triangle is read as 64-byte vectors from the constant address space
+// (s_load_dwordx16), two per block of 16 (FP64) or 32 (FP32) coefficients.
+// Empty asm statements order the stream without touching it: the fence at
+// the start of block b+1 names block b+1's registers (so the compiler's own
+// wait for them lands there, after block b's FMAs), the running sums (so
+// block b's FMAs cannot move past it) and the block pointer (so block b+2's
+// loads, addressed from it, cannot be hoisted above it); a scheduling
+// barrier after those loads keeps them ahead of block b+1's FMAs.  Left alone, the
+// compiler hoists every load of the unrolled contraction to its top and
+// spills the SGPRs into VGPR lanes.  A block may read up to 124 bytes past
+// the coefficient array; the tree handle pads its buffers for that.
 template <typename T>
 struct Chunk;
 template <>
 struct Chunk<double> {
-  typedef double V __attribute__((ext_vector_type(8)));
+  typedef double V __attribute__((ext_vector_type(8), aligned(8)));
   static constexpr int N = 8;
 };
 template <>
 struct Chunk<float> {
-  typedef float V __attribute__((ext_vector_type(16)));
+  typedef float V __attribute__((ext_vector_type(16), aligned(8)));
   static constexpr int N = 16;
 };
 
-template <typename T, int OFF>
-__device__ __forceinline__ typename Chunk<T>::V sload_chunk(const T *base) {
-  typename Chunk<T>::V r;
-  asm volatile("s_load_dwordx16 %0, %1, %2" : "=s"(r) : "s"(base), "i"(OFF));
-  return r;
+template <typename T>
+__device__ __forceinline__ typename Chunk<T>::V load_chunk(const T *p) {
+  typedef typename Chunk<T>::V V;
+  return *(const __attribute__((address_space(4))) V *)(p);
 }
 
-// Request both chunks of a block; the running sums pass through so that the
-// FMAs of the block being computed are not moved above the request.  The
-// destinations are early-clobber: the second load reads the base address
-// after the first one was issued, so the first destination must not share
-// registers with the base (a load returning early would overwrite it).
-template <typename T, typename V>
-__device__ __forceinline__ void sload_block(const T *base, V &a, V &b, T &w, T &v, T &u) {
-  constexpr int OFF1 = Chunk<T>::N * (int)sizeof(T);
-  asm volatile("s_load_dwordx16 %0, %5, 0\n\ts_load_dwordx16 %1, %5, %6"
-               : "=&s"(a), "=&s"(b), "+v"(w), "+v"(v), "+v"(u)
-               : "s"(base), "i"(OFF1));
+template <typename V, typename P>
+__device__ __forceinline__ void fence_block(V &a, V &b, P &ptr) {
+  asm volatile("" : "+s"(a), "+s"(b), "+s"(ptr));
 }
 
-template <typename V>
-__device__ __forceinline__ void wait_chunks(V &a, V &b) {
-  asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(a), "+s"(b));
-}
-
-// The same wait, also passing the running sums through: FMAs feeding them
-// must complete before it and FMAs after it cannot be hoisted above it, so
-// each block's FMAs stay between its own wait and the next one (otherwise
-// the compiler sinks all FMAs below all loads and every buffer stays live).
-template <typename V, typename T>
-__device__ __forceinline__ void wait_chunks(V &a, V &b, T &w, T &v, T &u) {
-  asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(a), "+s"(b), "+v"(w), "+v"(v), "+v"(u));
+template <typename V, typename P, typename T>
+__device__ __forceinline__ void fence_block(V &a, V &b, P &ptr, T &w, T &v, T &u) {
+  asm volatile("" : "+s"(a), "+s"(b), "+s"(ptr), "+v"(w), "+v"(v), "+v"(u));
 }
 
 template <int D, int MODE, typename T>
@@ -203,10 +187,15 @@ __device__ __forceinline__ T contract(const T *c, const T *px, const T *py,
   constexpr int BN = 2 * CN;       // coefficients per block
   constexpr int NC = D * (D + 1) * (D + 2) / 6;
   constexpr int NB = (NC + BN - 1) / BN;
-  constexpr int OFF1 = CN * (int)sizeof(T);
-  V cur0 = sload_chunk<T, 0>(c), cur1 = sload_chunk<T, OFF1>(c);
+  const T *bp = c;
+  V cur0 = load_chunk(bp), cur1 = load_chunk(bp + CN);
+  fence_block(cur0, cur1, bp);
   V nxt0 = cur0, nxt1 = cur1;
-  wait_chunks(cur0, cur1);
+  if constexpr (NB > 1) {
+    nxt0 = load_chunk(bp + BN);
+    nxt1 = load_chunk(bp + BN + CN);
+    __builtin_amdgcn_sched_barrier(0);  // issue them before block 0's FMAs
+  }
   T u = T(0);
   int idx = 0;
 #pragma unroll
@@ -217,16 +206,16 @@ __device__ __forceinline__ T contract(const T *c, const T *px, const T *py,
       T w = T(0);
 #pragma unroll
       for (int k = 0; i + j + k < D; ++k) {
-        if (idx % BN == 0) {
-          if (idx > 0) {
-            // block idx/BN was requested one block ago
-            wait_chunks(nxt0, nxt1, w, v, u);
-            cur0 = nxt0;
-            cur1 = nxt1;
-          }
+        if (idx % BN == 0 && idx > 0) {
+          // block idx/BN arrives here, after the previous block's FMAs
+          fence_block(nxt0, nxt1, bp, w, v, u);
+          cur0 = nxt0;
+          cur1 = nxt1;
+          bp = bp + BN;
           if (idx / BN + 1 < NB) {
-            // request the next block before this block's FMAs
-            sload_block(c + (idx / BN + 1) * BN, nxt0, nxt1, w, v, u);
+            nxt0 = load_chunk(bp + BN);
+            nxt1 = load_chunk(bp + BN + CN);
+            __builtin_amdgcn_sched_barrier(0);  // before this block's FMAs
           }
         }
         const int r = idx % BN;
@@ -340,27 +329,90 @@ __device__ __forceinline__ void eval_dof(const TreeView &t, double x, double y,
   });
 }
 
+// CubicInterpPolicy::InterpCubic1D (cubic.h:42-56) with its tangent
+// (cubic.h:36-39) and Hermite basis (cubic.h:28-34), in the reference's
+// operation order.
+__device__ __forceinline__ double cubic1d(double x, const double *xx, double p0,
+                                          double p1, double p2, double p3) {
+  const double mk = (p2 - p1) * 0.5 / (xx[2] - xx[1]) + (p1 - p0) * 0.5 / (xx[1] - xx[0]);
+  const double mk1 = (p3 - p2) * 0.5 / (xx[3] - xx[2]) + (p2 - p1) * 0.5 / (xx[2] - xx[1]);
+  const double t = (x - xx[1]) / (xx[2] - xx[1]);
+  const double h00 = 2 * t * t * t - 3 * t * t + 1;
+  const double h10 = t * t * t - 2 * t * t + t;
+  const double h01 = -2 * t * t * t + 3 * t * t;
+  const double h11 = t * t * t - t * t;
+  return h00 * p1 + h10 * (xx[2] - xx[1]) * mk + h01 * p2 + h11 * (xx[2] - xx[1]) * mk1;
+}
+
+// The velocity functor at one point for RK stage `stage` (1 or 2) at time
+// ts.  A single tree is contracted inline; several trees go through one
+// rolled loop (one contraction body) and are combined as the reference's
+// FieldExtrapFunctor (1.5 c - 0.5 p, tree_extrap_functor.h:71-77) or
+// FieldSetFunctor (cubic Hermite in time, tree_set_functor.h:66-72).
+template <int D, int MODE>
+__device__ __forceinline__ void eval_velocity(const VelSource &vs, int stage, double ts,
+                                              int scaled, double g, double x, double y,
+                                              double z, double &v0, double &v1, double &v2) {
+  if (vs.kind == kVelTree) {
+    eval_vec3<D, MODE>(vs.t[0], x, y, z, v0, v1, v2);
+    if (scaled) {
+      v0 = v0 * g;
+      v1 = v1 * g;
+      v2 = v2 * g;
+    }
+    return;
+  }
+  const int nt = vs.kind == kVelExtrap ? (stage == 1 ? 1 : 2) : 4;
+  double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+  double c0 = 0, c1 = 0, c2 = 0, d0 = 0, d1 = 0, d2 = 0;
+#pragma unroll 1
+  for (int k = 0; k < nt; ++k) {
+    const TreeView &tk = k == 0 ? vs.t[0] : k == 1 ? vs.t[1] : k == 2 ? vs.t[2] : vs.t[3];
+    double r0, r1, r2;
+    eval_vec3<D, MODE>(tk, x, y, z, r0, r1, r2);
+    if (k == 0) {
+      a0 = r0; a1 = r1; a2 = r2;
+    } else if (k == 1) {
+      b0 = r0; b1 = r1; b2 = r2;
+    } else if (k == 2) {
+      c0 = r0; c1 = r1; c2 = r2;
+    } else {
+      d0 = r0; d1 = r1; d2 = r2;
+    }
+  }
+  if (vs.kind == kVelExtrap) {
+    if (stage == 1) {
+      v0 = a0; v1 = a1; v2 = a2;
+    } else {
+      const double ccoeff = 3.0 / 2, pcoeff = 0.5;
+      v0 = ccoeff * a0 - pcoeff * b0;
+      v1 = ccoeff * a1 - pcoeff * b1;
+      v2 = ccoeff * a2 - pcoeff * b2;
+    }
+  } else {
+    v0 = cubic1d(ts, vs.tt, a0, b0, c0, d0);
+    v1 = cubic1d(ts, vs.tt, a1, b1, c1, d1);
+    v2 = cubic1d(ts, vs.tt, a2, b2, c2, d2);
+  }
+}
+
 // One IntegrateRK2 substep (traj.inc:20-46) of one point, in place.  The
 // first evaluation wraps x itself (EvalTree mutates the caller's array,
 // and ComputeTrajRK2 passes xsol as xinit); the midpoint is wrapped only
 // for its evaluation.
 template <int D, int MODE>
-__device__ __forceinline__ void rk2_substep(const TreeView &vel, int bc,
+__device__ __forceinline__ void rk2_substep(const VelSource &vel, int bc,
                                             double htau, double tau, int scaled,
-                                            double g1, double g2, double &x,
-                                            double &y, double &z) {
+                                            double g1, double g2, double ts1,
+                                            double ts2, double &x, double &y,
+                                            double &z) {
   if (bc == TBX_BC_PERIODIC) {
     x = wrap1(x);
     y = wrap1(y);
     z = wrap1(z);
   }
   double v0, v1, v2;
-  eval_vec3<D, MODE>(vel, x, y, z, v0, v1, v2);
-  if (scaled) {
-    v0 = v0 * g1;
-    v1 = v1 * g1;
-    v2 = v2 * g1;
-  }
+  eval_velocity<D, MODE>(vel, 1, ts1, scaled, g1, x, y, z, v0, v1, v2);
   // xtmp = xinit + (0.5 * dt) * vtmp   (traj.inc:36)
   double hx = x + htau * v0, hy = y + htau * v1, hz = z + htau * v2;
   if (bc == TBX_BC_PERIODIC) {
@@ -368,12 +420,7 @@ __device__ __forceinline__ void rk2_substep(const TreeView &vel, int bc,
     hy = wrap1(hy);
     hz = wrap1(hz);
   }
-  eval_vec3<D, MODE>(vel, hx, hy, hz, v0, v1, v2);
-  if (scaled) {
-    v0 = v0 * g2;
-    v1 = v1 * g2;
-    v2 = v2 * g2;
-  }
+  eval_velocity<D, MODE>(vel, 2, ts2, scaled, g2, hx, hy, hz, v0, v1, v2);
   // xsol = xinit + dt * vtmp   (traj.inc:41-42)
   x = x + tau * v0;
   y = y + tau * v1;

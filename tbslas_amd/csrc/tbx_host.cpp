@@ -320,7 +320,20 @@ hipError_t dispatch_eval(int q, int mode, const TreeView &t, double *pts,
   return hipErrorInvalidValue;
 }
 
-hipError_t dispatch_traj(int q, int mode, const TreeView &vel, double *x,
+hipError_t dispatch_veval(int q, int mode, const VelSource &vel, double *pts, int64_t n,
+                          int bc, int wb, double t, int scaled, double g, double *out,
+                          hipStream_t s) {
+  switch (q) {
+#define X(Q) \
+  case Q:    \
+    return launch_veval_q##Q(mode, vel, pts, n, bc, wb, t, scaled, g, out, s);
+    TBX_Q_CASES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t dispatch_traj(int q, int mode, const VelSource &vel, double *x,
                          int64_t n, const SubArgs &a, hipStream_t s) {
   switch (q) {
 #define X(Q) \
@@ -332,7 +345,7 @@ hipError_t dispatch_traj(int q, int mode, const TreeView &vel, double *x,
   return hipErrorInvalidValue;
 }
 
-hipError_t dispatch_step(int q, int mode, const TreeView &vel,
+hipError_t dispatch_step(int q, int mode, const VelSource &vel,
                          const TreeView &conc, const StepArgs &a, double *vals,
                          double *dep, hipStream_t s) {
   switch (q) {
@@ -364,10 +377,53 @@ SubArgs sub_args(double tcur, double tau, tbx_tfac_fn g, void *gctx, int bc) {
   a.tau = tau;
   a.htau = 0.5 * tau;
   a.scaled = g ? 1 : 0;
-  a.g1 = g ? g(tcur, gctx) : 1.0;
-  a.g2 = g ? g(tcur + 0.5 * tau, gctx) : 1.0;
+  a.ts1 = tcur;
+  a.ts2 = tcur + 0.5 * tau;
+  a.g1 = g ? g(a.ts1, gctx) : 1.0;
+  a.g2 = g ? g(a.ts2, gctx) : 1.0;
   a.bc = bc;
   return a;
+}
+
+// Checks a velocity functor and prepares its trees for the mode.
+int prepare_velocity(const tbx_velocity *v, int mode, hipStream_t s, VelSource *out,
+                     int *q, const char *who) {
+  if (!v) return fail(TBX_EINVAL, "%s: null velocity", who);
+  int nt;
+  switch (v->kind) {
+    case TBX_VEL_TREE: nt = 1; break;
+    case TBX_VEL_EXTRAP: nt = 2; break;
+    case TBX_VEL_SET: nt = 4; break;
+    default: return fail(TBX_EINVAL, "%s: unknown velocity kind %d", who, v->kind);
+  }
+  if (v->kind != TBX_VEL_TREE && v->g)
+    return fail(TBX_EINVAL, "%s: a time factor applies to a single velocity tree only", who);
+  *out = VelSource{};
+  for (int k = 0; k < nt; ++k) {
+    tbx_tree *t = const_cast<tbx_tree *>(v->trees[k]);
+    if (!t) return fail(TBX_EINVAL, "%s: velocity tree %d is null", who, k);
+    if (t->dof != 3) return fail(TBX_EINVAL, "%s: velocity tree %d has dof %d", who, k, t->dof);
+    if (k > 0 && t->q != v->trees[0]->q)
+      return fail(TBX_EUNSUPPORTED, "%s: velocity trees differ in degree", who);
+    int rc = prepare_mode(t, mode, s);
+    if (rc) return rc;
+    out->t[k] = t->view();
+  }
+  if (v->kind == TBX_VEL_SET)
+    for (int k = 0; k < 4; ++k) out->tt[k] = v->times[k];
+  out->kind = v->kind;
+  out->ntree = nt;
+  *q = v->trees[0]->q;
+  return TBX_OK;
+}
+
+tbx_velocity single_tree(const tbx_tree *t, tbx_tfac_fn g, void *gctx) {
+  tbx_velocity v{};
+  v.kind = TBX_VEL_TREE;
+  v.trees[0] = t;
+  v.g = g;
+  v.gctx = gctx;
+  return v;
 }
 
 int ensure_refit(tbx_tree *t, hipStream_t s) {
@@ -581,20 +637,37 @@ int tbx_eval(const tbx_tree *tc, double *pts, int64_t n, int bc, uint32_t flags,
   return TBX_OK;
 }
 
-int tbx_traj_rk2(const tbx_tree *velc, tbx_tfac_fn g, void *gctx, const double *xinit, int64_t n,
-                 double tinit, double tfinal, int num_rk_step, int bc, uint32_t flags,
-                 double *xsol, tbx_stream s) {
-  tbx_tree *vel = const_cast<tbx_tree *>(velc);
-  if (!vel) return fail(TBX_EINVAL, "tbx_traj_rk2: null tree");
-  if (vel->dof != 3) return fail(TBX_EINVAL, "tbx_traj_rk2: velocity tree has dof %d", vel->dof);
+int tbx_velocity_eval(const tbx_velocity *vel, double *pts, int64_t n, double t, int bc,
+                      uint32_t flags, double *out, tbx_stream s) {
+  int rc = check_points(pts, n, "tbx_velocity_eval");
+  if (rc) return rc;
+  if ((rc = check_bc(bc))) return rc;
+  if (n > 0 && !out) return fail(TBX_EINVAL, "tbx_velocity_eval: null output");
+  int mode, q;
+  if ((rc = mode_of(flags, &mode))) return rc;
+  VelSource vs;
+  if ((rc = prepare_velocity(vel, mode, (hipStream_t)s, &vs, &q, "tbx_velocity_eval"))) return rc;
+  if (n == 0) return TBX_OK;
+  // The extrapolating functor's value is its second-stage form, the
+  // FieldExtrapFunctor itself (tree_extrap_functor.h:47-78).
+  HIP_TRY(dispatch_veval(q, mode, vs, pts, n, bc, (flags & TBX_WRITE_BACK_WRAP) ? 1 : 0, t,
+                         vel->g ? 1 : 0, vel->g ? vel->g(t, vel->gctx) : 1.0, out,
+                         (hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_traj_rk2_v(const tbx_velocity *vel, const double *xinit, int64_t n, double tinit,
+                   double tfinal, int num_rk_step, int bc, uint32_t flags, double *xsol,
+                   tbx_stream s) {
   int rc = check_points(xinit, n, "tbx_traj_rk2");
   if (rc) return rc;
   if ((rc = check_bc(bc))) return rc;
   if (num_rk_step <= 0) return fail(TBX_EINVAL, "tbx_traj_rk2: num_rk_step %d", num_rk_step);
   if (n > 0 && !xsol) return fail(TBX_EINVAL, "tbx_traj_rk2: null output");
-  int mode;
+  int mode, q;
   if ((rc = mode_of(flags, &mode))) return rc;
-  if ((rc = prepare_mode(vel, mode, (hipStream_t)s))) return rc;
+  VelSource vs;
+  if ((rc = prepare_velocity(vel, mode, (hipStream_t)s, &vs, &q, "tbx_traj_rk2"))) return rc;
   if (n == 0) return TBX_OK;
   if (xsol != xinit)
     HIP_TRY(hipMemcpyAsync(xsol, xinit, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice,
@@ -602,11 +675,19 @@ int tbx_traj_rk2(const tbx_tree *velc, tbx_tfac_fn g, void *gctx, const double *
   const double tau = (tfinal - tinit) / num_rk_step;  // traj.inc:55
   double tcur = tinit;
   for (int k = 0; k < num_rk_step; ++k) {
-    const SubArgs a = sub_args(tcur, tau, g, gctx, bc);
-    HIP_TRY(dispatch_traj(vel->q, mode, vel->view(), xsol, n, a, (hipStream_t)s));
+    const SubArgs a = sub_args(tcur, tau, vel->g, vel->gctx, bc);
+    HIP_TRY(dispatch_traj(q, mode, vs, xsol, n, a, (hipStream_t)s));
     tcur = tcur + tau;  // traj.inc:65
   }
   return TBX_OK;
+}
+
+int tbx_traj_rk2(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, const double *xinit, int64_t n,
+                 double tinit, double tfinal, int num_rk_step, int bc, uint32_t flags,
+                 double *xsol, tbx_stream s) {
+  if (!vel) return fail(TBX_EINVAL, "tbx_traj_rk2: null tree");
+  const tbx_velocity v = single_tree(vel, g, gctx);
+  return tbx_traj_rk2_v(&v, xinit, n, tinit, tfinal, num_rk_step, bc, flags, xsol, s);
 }
 
 int tbx_semilag_rk2(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, const tbx_tree *concc,
@@ -644,23 +725,22 @@ int tbx_set_grid_values(tbx_tree *t, const double *vals, tbx_stream s) {
 namespace {
 
 // Fused arrival -> RK2 -> conc evaluation into vals [L][dof][P].
-int step_values(tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc, int timestep,
-                double dt, int num_rk_step, int bc, uint32_t flags, double *vals, double *dep,
+int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt,
+                int num_rk_step, int bc, uint32_t flags, double *vals, double *dep,
                 hipStream_t st, const char *who) {
-  if (!vel || !conc) return fail(TBX_EINVAL, "%s: null tree", who);
+  if (!conc) return fail(TBX_EINVAL, "%s: null tree", who);
   if (!vals) return fail(TBX_EINVAL, "%s: null value buffer", who);
-  if (vel->dof != 3) return fail(TBX_EINVAL, "%s: velocity dof %d", who, vel->dof);
-  if (vel->q != conc->q)
-    return fail(TBX_EUNSUPPORTED, "%s: velocity q %d != concentration q %d", who, vel->q,
-                conc->q);
   if (num_rk_step <= 0 || num_rk_step > kMaxSub)
     return fail(TBX_EUNSUPPORTED, "%s: num_rk_step %d outside 1..%d", who, num_rk_step,
                 kMaxSub);
   int rc = check_bc(bc);
   if (rc) return rc;
-  int mode;
+  int mode, q;
   if ((rc = mode_of(flags, &mode))) return rc;
-  if ((rc = prepare_mode(vel, mode, st))) return rc;
+  VelSource vs;
+  if ((rc = prepare_velocity(vel, mode, st, &vs, &q, who))) return rc;
+  if (q != conc->q)
+    return fail(TBX_EUNSUPPORTED, "%s: velocity q %d != concentration q %d", who, q, conc->q);
   if ((rc = prepare_mode(conc, mode, st))) return rc;
   const int P = conc->P();
   StepArgs a{};
@@ -673,42 +753,64 @@ int step_values(tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc, int ti
   a.P = P;
   a.nsub = num_rk_step;
   a.bc = bc;
-  a.scaled = g ? 1 : 0;
+  a.scaled = vel->g ? 1 : 0;
   const double tinit = timestep * dt, tfinal = tinit - dt;  // semilag.inc:34-35
   a.tau = (tfinal - tinit) / num_rk_step;                    // traj.inc:55
   a.htau = 0.5 * a.tau;
   double tcur = tinit;
   for (int k = 0; k < num_rk_step; ++k) {
-    a.g[2 * k] = g ? g(tcur, gctx) : 1.0;
-    a.g[2 * k + 1] = g ? g(tcur + 0.5 * a.tau, gctx) : 1.0;
+    a.ts[2 * k] = tcur;
+    a.ts[2 * k + 1] = tcur + 0.5 * a.tau;
+    a.g[2 * k] = vel->g ? vel->g(a.ts[2 * k], vel->gctx) : 1.0;
+    a.g[2 * k + 1] = vel->g ? vel->g(a.ts[2 * k + 1], vel->gctx) : 1.0;
     tcur = tcur + a.tau;
   }
   new_nodes_host(conc->q, a.node);
-  HIP_TRY(dispatch_step(conc->q, mode, vel->view(), conc->view(), a, vals, dep, st));
+  HIP_TRY(dispatch_step(conc->q, mode, vs, conc->view(), a, vals, dep, st));
   return TBX_OK;
+}
+
+int step_insitu(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt,
+                int num_rk_step, int bc, uint32_t flags, double *dep, hipStream_t st,
+                const char *who) {
+  if (!conc) return fail(TBX_EINVAL, "%s: null tree", who);
+  if (!conc->d_vals)
+    HIP_TRY(hipMalloc(&conc->d_vals, sizeof(double) * (size_t)conc->L * conc->dof * conc->P()));
+  int rc = step_values(vel, conc, timestep, dt, num_rk_step, bc, flags, conc->d_vals, dep, st,
+                       who);
+  if (rc) return rc;
+  return refit(conc, conc->d_vals, st);
 }
 
 }  // namespace
 
+int tbx_semilag_values_v(const tbx_velocity *vel, const tbx_tree *conc, int timestep, double dt,
+                         int num_rk_step, int bc, uint32_t flags, double *vals, double *dep,
+                         tbx_stream s) {
+  return step_values(vel, const_cast<tbx_tree *>(conc), timestep, dt, num_rk_step, bc, flags,
+                     vals, dep, (hipStream_t)s, "tbx_semilag_values");
+}
+
 int tbx_semilag_values(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, const tbx_tree *conc,
                        int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
                        double *vals, double *dep, tbx_stream s) {
-  return step_values(const_cast<tbx_tree *>(vel), g, gctx, const_cast<tbx_tree *>(conc),
-                     timestep, dt, num_rk_step, bc, flags, vals, dep, (hipStream_t)s,
-                     "tbx_semilag_values");
+  if (!vel) return fail(TBX_EINVAL, "tbx_semilag_values: null tree");
+  const tbx_velocity v = single_tree(vel, g, gctx);
+  return tbx_semilag_values_v(&v, conc, timestep, dt, num_rk_step, bc, flags, vals, dep, s);
+}
+
+int tbx_semilag_insitu_v(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt,
+                         int num_rk_step, int bc, uint32_t flags, double *dep, tbx_stream s) {
+  return step_insitu(vel, conc, timestep, dt, num_rk_step, bc, flags, dep, (hipStream_t)s,
+                     "tbx_semilag_insitu");
 }
 
 int tbx_semilag_insitu(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc,
                        int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
                        double *dep, tbx_stream s) {
-  if (!conc) return fail(TBX_EINVAL, "tbx_semilag_insitu: null tree");
-  hipStream_t st = (hipStream_t)s;
-  if (!conc->d_vals)
-    HIP_TRY(hipMalloc(&conc->d_vals, sizeof(double) * (size_t)conc->L * conc->dof * conc->P()));
-  int rc = step_values(const_cast<tbx_tree *>(vel), g, gctx, conc, timestep, dt, num_rk_step,
-                       bc, flags, conc->d_vals, dep, st, "tbx_semilag_insitu");
-  if (rc) return rc;
-  return refit(conc, conc->d_vals, st);
+  if (!vel) return fail(TBX_EINVAL, "tbx_semilag_insitu: null tree");
+  const tbx_velocity v = single_tree(vel, g, gctx);
+  return tbx_semilag_insitu_v(&v, conc, timestep, dt, num_rk_step, bc, flags, dep, s);
 }
 
 int tbx_set_option(int option, int value) {

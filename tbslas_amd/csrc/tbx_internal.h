@@ -32,6 +32,22 @@ struct TreeView {
   int32_t dir_shift;     // 3 * (21 - g)
 };
 
+// A velocity functor of the kinds the reference's trajectory templates take.
+//  kVelTree   NodeFieldFunctor(t[0]), optionally scaled by g(t);
+//  kVelExtrap field_fn = NodeFieldFunctor(t[0]) for the first RK stage and
+//             extrap_fn = FieldExtrapFunctor(t[1], t[0]) for the second
+//             (traj.inc:70-95, tree_extrap_functor.h:47-78): t[0] current,
+//             t[1] previous;
+//  kVelSet    FieldSetFunctor(t[0..3], tt[0..3]) (tree_set_functor.h:49-79).
+enum VelKind : int { kVelTree = 0, kVelExtrap = 1, kVelSet = 2 };
+
+struct VelSource {
+  TreeView t[4];
+  double tt[4];
+  int32_t kind;
+  int32_t ntree;
+};
+
 constexpr int kMaxSub = 16;  // RK substeps fused into one step launch
 
 // One IntegrateRK2 substep: stage factors of the time modulation.
@@ -40,6 +56,8 @@ struct SubArgs {
   double htau;  // 0.5 * tau                         (traj.inc:36)
   double g1;    // g(t)            (1 when steady)
   double g2;    // g(t + 0.5 tau)
+  double ts1;   // stage times t and t + 0.5 tau (traj.inc:33, 40)
+  double ts2;
   int scaled;   // multiply by g1/g2
   int bc;
 };
@@ -62,6 +80,7 @@ struct StepArgs {
   int32_t scaled;
   double tau, htau;
   double g[2 * kMaxSub];
+  double ts[2 * kMaxSub];  // stage times
   double node[TBX_MAX_Q + 1];  // stretched nodes (cheb.h:53-58)
 };
 
@@ -72,11 +91,15 @@ constexpr int kBlock = 256;
   hipError_t launch_eval_q##Q(int mode, const TreeView &t, double *pts,       \
                               int64_t n, int bc, int write_back, double *out, \
                               hipStream_t s);                                 \
-  hipError_t launch_traj_q##Q(int mode, const TreeView &vel, double *x,       \
+  hipError_t launch_traj_q##Q(int mode, const VelSource &vel, double *x,      \
                               int64_t n, const SubArgs &a, hipStream_t s);    \
-  hipError_t launch_step_q##Q(int mode, const TreeView &vel,                  \
+  hipError_t launch_step_q##Q(int mode, const VelSource &vel,                 \
                               const TreeView &conc, const StepArgs &a,        \
-                              double *vals, double *dep, hipStream_t s);
+                              double *vals, double *dep, hipStream_t s);      \
+  hipError_t launch_veval_q##Q(int mode, const VelSource &vel, double *pts,   \
+                               int64_t n, int bc, int write_back,             \
+                               double t, int scaled, double g, double *out,   \
+                               hipStream_t s);
 
 TBX_DECLARE_Q(1)
 TBX_DECLARE_Q(2)

@@ -40,16 +40,46 @@ __global__ __launch_bounds__(kBlock) void k_eval(TreeView t,
   eval_dof<D, MODE>(t, x, y, z, [&](int l, double v) { o[l] = v; });
 }
 
+// A velocity functor (FieldExtrapFunctor / FieldSetFunctor / a scaled tree)
+// at caller points at time t; the extrapolating kind evaluates its
+// extrapolation (the second-stage functor).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_veval(VelSource vel,
+                                                  double *__restrict__ pts,
+                                                  int64_t n, int bc,
+                                                  int write_back, double t,
+                                                  int scaled, double g,
+                                                  double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double x = pts[3 * i + 0], y = pts[3 * i + 1], z = pts[3 * i + 2];
+  if (bc == TBX_BC_PERIODIC) {
+    x = wrap1(x);
+    y = wrap1(y);
+    z = wrap1(z);
+    if (write_back) {
+      pts[3 * i + 0] = x;
+      pts[3 * i + 1] = y;
+      pts[3 * i + 2] = z;
+    }
+  }
+  double v0, v1, v2;
+  eval_velocity<D, MODE>(vel, 2, t, scaled, g, x, y, z, v0, v1, v2);
+  out[3 * i + 0] = v0;
+  out[3 * i + 1] = v1;
+  out[3 * i + 2] = v2;
+}
+
 // One IntegrateRK2 substep for caller points, in place.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_traj(TreeView vel,
+__global__ __launch_bounds__(kBlock) void k_traj(VelSource vel,
                                                  double *__restrict__ xs,
                                                  int64_t n, SubArgs a) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   double x = xs[3 * i + 0], y = xs[3 * i + 1], z = xs[3 * i + 2];
-  rk2_substep<D, MODE>(vel, a.bc, a.htau, a.tau, a.scaled, a.g1, a.g2, x, y,
-                       z);
+  rk2_substep<D, MODE>(vel, a.bc, a.htau, a.tau, a.scaled, a.g1, a.g2, a.ts1,
+                       a.ts2, x, y, z);
   xs[3 * i + 0] = x;
   xs[3 * i + 1] = y;
   xs[3 * i + 2] = z;
@@ -65,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_traj(TreeView vel,
 // leaves, so their scalar coefficient loads hit the same cache lines.
 template <int MODE, int BLOCK, int WPE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_step(
-    TreeView vel, TreeView conc, StepArgs a, double *__restrict__ vals,
+    VelSource vel, TreeView conc, StepArgs a, double *__restrict__ vals,
     double *__restrict__ dep) {
   const uint32_t b = blockIdx.x;
   const int64_t lb = (int64_t)(b & 7u) * a.per_xcd + (int64_t)(b >> 3);
@@ -83,7 +113,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
   // ComputeTrajRK2 (traj.inc:48-68), tinit -> tfinal = tinit - dt
   for (int s = 0; s < a.nsub; ++s)
     rk2_substep<D, MODE>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[2 * s],
-                         a.g[2 * s + 1], x, y, z);
+                         a.g[2 * s + 1], a.ts[2 * s], a.ts[2 * s + 1], x, y, z);
   // con_evaluator(departure points) wraps them in place (semilag.inc:43)
   if (a.bc == TBX_BC_PERIODIC) {
     x = wrap1(x);
@@ -135,7 +165,32 @@ hipError_t TBX_CAT(launch_eval_q, TBX_Q)(int mode, const TreeView &t,
   return hipGetLastError();
 }
 
-hipError_t TBX_CAT(launch_traj_q, TBX_Q)(int mode, const TreeView &vel,
+hipError_t TBX_CAT(launch_veval_q, TBX_Q)(int mode, const VelSource &vel,
+                                          double *pts, int64_t n, int bc,
+                                          int write_back, double t, int scaled,
+                                          double g, double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 gr(grid_for(n)), blk(kBlock);
+  switch (mode) {
+    case kFma64:
+      hipLaunchKernelGGL(k_veval<kFma64>, gr, blk, 0, s, vel, pts, n, bc, write_back, t,
+                         scaled, g, out);
+      break;
+    case kExact64:
+      hipLaunchKernelGGL(k_veval<kExact64>, gr, blk, 0, s, vel, pts, n, bc, write_back, t,
+                         scaled, g, out);
+      break;
+    case kFp32:
+      hipLaunchKernelGGL(k_veval<kFp32>, gr, blk, 0, s, vel, pts, n, bc, write_back, t,
+                         scaled, g, out);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t TBX_CAT(launch_traj_q, TBX_Q)(int mode, const VelSource &vel,
                                          double *x, int64_t n,
                                          const SubArgs &a, hipStream_t s) {
   if (n <= 0) return hipSuccess;
@@ -159,7 +214,7 @@ hipError_t TBX_CAT(launch_traj_q, TBX_Q)(int mode, const TreeView &vel,
 namespace {
 
 template <int MODE, int BLOCK, int WPE>
-hipError_t launch_step_shape(const TreeView &vel, const TreeView &conc, const StepArgs &a,
+hipError_t launch_step_shape(const VelSource &vel, const TreeView &conc, const StepArgs &a,
                              double *vals, double *dep, hipStream_t s) {
   hipLaunchKernelGGL((k_step<MODE, BLOCK, WPE>), dim3((unsigned)(8 * a.per_xcd)), dim3(BLOCK), 0,
                      s, vel, conc, a, vals, dep);
@@ -167,7 +222,7 @@ hipError_t launch_step_shape(const TreeView &vel, const TreeView &conc, const St
 }
 
 template <int MODE>
-hipError_t launch_step_mode(const TreeView &vel, const TreeView &conc, const StepArgs &a,
+hipError_t launch_step_mode(const VelSource &vel, const TreeView &conc, const StepArgs &a,
                             double *vals, double *dep, hipStream_t s) {
   // launch shapes of kStepShapes (tbx_internal.h); the extra shapes are
   // built for the degrees of the benchmark workloads only
@@ -187,7 +242,7 @@ hipError_t launch_step_mode(const TreeView &vel, const TreeView &conc, const Ste
 
 }  // namespace
 
-hipError_t TBX_CAT(launch_step_q, TBX_Q)(int mode, const TreeView &vel,
+hipError_t TBX_CAT(launch_step_q, TBX_Q)(int mode, const VelSource &vel,
                                          const TreeView &conc,
                                          const StepArgs &a, double *vals,
                                          double *dep, hipStream_t s) {

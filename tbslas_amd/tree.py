@@ -16,7 +16,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import BC_FREESPACE, BC_PERIODIC, MODES, WRITE_BACK_WRAP, check, lib, tfac
+from ._lib import (BC_FREESPACE, BC_PERIODIC, MODES, VEL_EXTRAP, VEL_SET, VEL_TREE,
+                   WRITE_BACK_WRAP, _Velocity, check, lib, tfac)
 
 
 def ncoef(q: int) -> int:
@@ -190,16 +191,86 @@ class Tree:
         check(lib().tbx_set_grid_values(self._h, _ptr(vals), _stream()))
 
 
-def traj_rk2(vel: Tree, xinit, tinit: float, tfinal: float, num_rk_step: int,
+class Velocity:
+    """A velocity functor over dof-3 device trees of one degree.
+
+    Velocity.tree(t, g)        NodeFieldFunctor (tree_functor.h:793-815),
+                               scaled by g(t) when g is given;
+    Velocity.extrap(cur, prev) the extrapolating RK2 overloads
+                               (traj.inc:70-115): cur for the first stage,
+                               FieldExtrapFunctor(prev, cur) =
+                               1.5 cur - 0.5 prev for the second
+                               (tree_extrap_functor.h:47-78);
+    Velocity.set(trees, times) FieldSetFunctor (tree_set_functor.h:49-79),
+                               cubic Hermite in time over four trees.
+    """
+
+    def __init__(self, kind: int, trees, times=None, g=None):
+        n = {VEL_TREE: 1, VEL_EXTRAP: 2, VEL_SET: 4}[kind]
+        if len(trees) != n:
+            raise ValueError(f"velocity kind {kind} takes {n} trees")
+        self.trees = list(trees)
+        self._cb = tfac(g)
+        self._s = _Velocity()
+        self._s.kind = kind
+        for i, t in enumerate(self.trees):
+            self._s.trees[i] = t.handle
+        if times is not None:
+            if len(times) != 4:
+                raise ValueError("a velocity set takes four times")
+            for i, x in enumerate(times):
+                self._s.times[i] = float(x)
+        self._s.g = self._cb
+
+    @classmethod
+    def tree(cls, t: "Tree", g=None) -> "Velocity":
+        return cls(VEL_TREE, [t], g=g)
+
+    @classmethod
+    def extrap(cls, cur: "Tree", prev: "Tree") -> "Velocity":
+        return cls(VEL_EXTRAP, [cur, prev])
+
+    @classmethod
+    def set(cls, trees, times) -> "Velocity":
+        return cls(VEL_SET, trees, times)
+
+    @property
+    def ref(self):
+        return C.byref(self._s)
+
+    def eval(self, pts, t: float, bc: int = BC_FREESPACE, mode: str = "fp64",
+             write_back: bool = False):
+        """The functor's value at pts [n][3] at time t (the extrapolating
+        functor gives its FieldExtrapFunctor value)."""
+        import torch
+
+        _require_dev(pts, torch.float64, "pts", (3,))
+        out = torch.empty((pts.shape[0], 3), dtype=torch.float64, device=pts.device)
+        flags = _mode(mode) | (WRITE_BACK_WRAP if write_back else 0)
+        check(lib().tbx_velocity_eval(self.ref, _ptr(pts), pts.shape[0], t, bc, flags,
+                                      _ptr(out), _stream()))
+        return out
+
+
+def _velocity(vel, g) -> Velocity:
+    if isinstance(vel, Velocity):
+        if g is not None:
+            raise ValueError("pass g to Velocity.tree, not with a Velocity")
+        return vel
+    return Velocity.tree(vel, g)
+
+
+def traj_rk2(vel, xinit, tinit: float, tfinal: float, num_rk_step: int,
              bc: int = BC_FREESPACE, mode: str = "fp64", g=None, out=None):
-    """ComputeTrajRK2 (traj.inc:48-68) through vel's functor."""
+    """ComputeTrajRK2 (traj.inc:48-68, 97-115) through vel's functor (a Tree,
+    optionally scaled by g, or a Velocity)."""
     import torch
 
+    v = _velocity(vel, g)
     _require_dev(xinit, torch.float64, "xinit", (3,))
     xsol = torch.empty_like(xinit) if out is None else out
-    cb = tfac(g)
-    check(lib().tbx_traj_rk2(vel.handle, cb, None, _ptr(xinit), xinit.shape[0], tinit, tfinal,
-                             num_rk_step, bc, _mode(mode), _ptr(xsol), _stream()))
+    check(lib().tbx_traj_rk2_v(v.ref, _ptr(xinit), xinit.shape[0], tinit, tfinal, num_rk_step,
+                               bc, _mode(mode), _ptr(xsol), _stream()))
     return xsol
 
 
@@ -217,29 +288,29 @@ def semilag_rk2(vel: Tree, conc: Tree, pts, timestep: int, dt: float, num_rk_ste
     return vals
 
 
-def semilag_insitu(vel: Tree, conc: Tree, timestep: int, dt: float, num_rk_step: int = 1,
+def semilag_insitu(vel, conc: Tree, timestep: int, dt: float, num_rk_step: int = 1,
                    bc: int = BC_FREESPACE, mode: str = "fp64", g=None, dep=None):
-    """SolveSemilagInSitu (tree_semilag.h:92-135): one fused step, conc refit
+    """SolveSemilagInSitu (tree_semilag.h:92-181): one fused step, conc refit
     in place.  dep (optional cuda tensor [L*P][3]) receives departure points."""
-    cb = tfac(g)
-    check(lib().tbx_semilag_insitu(vel.handle, cb, None, conc.handle, timestep, dt, num_rk_step,
-                                   bc, _mode(mode), _ptr(dep), _stream()))
+    v = _velocity(vel, g)
+    check(lib().tbx_semilag_insitu_v(v.ref, conc.handle, timestep, dt, num_rk_step, bc,
+                                     _mode(mode), _ptr(dep), _stream()))
 
 
-def semilag_values(vel: Tree, conc: Tree, timestep: int, dt: float, num_rk_step: int = 1,
+def semilag_values(vel, conc: Tree, timestep: int, dt: float, num_rk_step: int = 1,
                    bc: int = BC_FREESPACE, mode: str = "fp64", g=None, vals=None, dep=None):
     """SolveSemilagInSitu without the refit: conc at the departure points of
     its own grid points, returned as [L][dof][P] (the set_grid_values layout)."""
     import torch
 
+    v = _velocity(vel, g)
     if vals is None:
         vals = torch.empty((conc.L, conc.dof, conc.P), dtype=torch.float64, device="cuda")
     _require_dev(vals, torch.float64, "vals")
     if vals.numel() != conc.L * conc.dof * conc.P:
         raise ValueError("vals must hold L*dof*P values")
-    cb = tfac(g)
-    check(lib().tbx_semilag_values(vel.handle, cb, None, conc.handle, timestep, dt, num_rk_step,
-                                   bc, _mode(mode), _ptr(vals), _ptr(dep), _stream()))
+    check(lib().tbx_semilag_values_v(v.ref, conc.handle, timestep, dt, num_rk_step, bc,
+                                     _mode(mode), _ptr(vals), _ptr(dep), _stream()))
     return vals
 
 

@@ -244,3 +244,96 @@ def test_c1_full_size_step(torch, tb, orc):
     leaf_ref = orc.locate(oc, ref_dep, 0)
     assert (gc.locate(dep, 0).cpu().numpy() == leaf_ref).all()
     assert rel(gc.get_coeffs(), oc.coeff) <= FP64_TOL
+
+
+def _vel_trees(tb, orc, anchor, depth, q, n, seed):
+    # n distinct smooth dof-3 fields on one tree: Taylor-Green scaled and
+    # shifted per tree, so every tree differs
+    from tbslas_amd import fields
+
+    out = []
+    for k in range(n):
+        s, o = 1.0 - 0.15 * k, 0.05 * (k + seed)
+        cf = tb.fit_function(lambda p, s=s, o=o: s * fields.taylor_green_2d3(p) + o, anchor, depth,
+                             q, 3)
+        out.append(pair(tb, orc, anchor, depth, q, 3, cf))
+    return [g for g, _ in out], [o for _, o in out]
+
+
+@pytest.mark.parametrize("kind", ["extrap", "set"])
+def test_velocity_functors_against_oracle(torch, tb, orc, kind):
+    anchor, depth = random_tree(31, max_depth=4)
+    q = 6
+    n = 2 if kind == "extrap" else 4
+    gs, os_ = _vel_trees(tb, orc, anchor, depth, q, n, 1)
+    tt = [0.1, 0.2, 0.35, 0.5]
+    if kind == "extrap":
+        gvel, ovel = tb.Velocity.extrap(gs[0], gs[1]), orc.velocity(orc.VEL_EXTRAP, os_)
+    else:
+        gvel, ovel = tb.Velocity.set(gs, tt), orc.velocity(orc.VEL_SET, os_, tt)
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.random((20000, 3)), edge_points()])
+    for bc in (0, 1):
+        xx = x * 1.2 - 0.1 if bc else x
+        # functor values
+        if kind == "extrap":
+            ref = orc.field_extrap_eval(os_[1], os_[0], xx, bc)
+        else:
+            ref = orc.field_set_eval(os_, tt, xx, 0.27, bc)
+        got = gvel.eval(dev(torch, xx), 0.27, bc, "exact").cpu().numpy()
+        assert np.array_equal(got, ref)
+        got = gvel.eval(dev(torch, xx), 0.27, bc, "fp64").cpu().numpy()
+        assert rel(got, ref) <= FP64_TOL
+        # trajectories, both stages
+        ref = orc.traj_rk2_v(ovel, xx, 0.4, 0.3, 2, bc)
+        got = tb.traj_rk2(gvel, dev(torch, xx), 0.4, 0.3, 2, bc, "exact").cpu().numpy()
+        assert np.array_equal(got, ref)
+        got = tb.traj_rk2(gvel, dev(torch, xx), 0.4, 0.3, 2, bc, "fp64").cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-13
+
+
+@pytest.mark.parametrize("kind,mode", [("extrap", "exact"), ("set", "exact"), ("set", "fp64")])
+def test_insitu_velocity_functors_against_oracle(torch, tb, orc, kind, mode):
+    from tbslas_amd import fields
+
+    q = 6
+    anchor, depth = random_tree(22, max_depth=3)
+    n = 2 if kind == "extrap" else 4
+    gs, os_ = _vel_trees(tb, orc, anchor, depth, q, n, 2)
+    tt = [0.0, 0.03, 0.06, 0.09]
+    if kind == "extrap":
+        gvel, ovel = tb.Velocity.extrap(gs[0], gs[1]), orc.velocity(orc.VEL_EXTRAP, os_)
+    else:
+        gvel, ovel = tb.Velocity.set(gs, tt), orc.velocity(orc.VEL_SET, os_, tt)
+    cf = tb.fit_function(lambda p: fields.gaussian(p, (0.6, 0.45, 0.5), 0.1), anchor, depth, q, 1)
+    gc, oc = pair(tb, orc, anchor, depth, q, 1, cf)
+    M = orc.pt2coeff(q)
+    dt = 0.02
+    dep = torch.empty((gc.L * gc.P, 3), dtype=torch.float64, device="cuda")
+    for step in range(2, 5):
+        pts = orc.collect_grid_points(oc)
+        ref_dep = orc.traj_rk2_v(ovel, pts, step * dt, step * dt - dt, 2, 1)
+        ref_dep = np.where(ref_dep < 0, ref_dep + 1, ref_dep)
+        ref_dep = np.where(ref_dep >= 1, ref_dep - 1, ref_dep)
+        orc.semilag_insitu_v(ovel, oc, M, step, dt, 2, 1)
+        tb.semilag_insitu(gvel, gc, step, dt, 2, 1, mode, dep=dep)
+        if mode == "exact":
+            assert np.array_equal(dep.cpu().numpy(), ref_dep)
+        else:
+            assert np.abs(dep.cpu().numpy() - ref_dep).max() <= 1e-14
+        err = np.abs(gc.get_coeffs() - oc.coeff).max() / np.abs(oc.coeff).max()
+        assert err <= FP64_TOL, (step, err)
+
+
+def test_velocity_errors(torch, tb):
+    anchor, depth = tb.uniform_leaves(1)
+    v3 = tb.Tree(anchor, depth, 4, 3)
+    v3b = tb.Tree(anchor, depth, 5, 3)
+    s1 = tb.Tree(anchor, depth, 4, 1)
+    x = torch.zeros((4, 3), dtype=torch.float64, device="cuda")
+    with pytest.raises(tb.TbxError):   # dof 1 velocity
+        tb.traj_rk2(tb.Velocity.extrap(v3, s1), x, 0.1, 0.0, 1)
+    with pytest.raises(tb.TbxError):   # degrees differ
+        tb.traj_rk2(tb.Velocity.extrap(v3, v3b), x, 0.1, 0.0, 1)
+    with pytest.raises(ValueError):
+        tb.Velocity.set([v3, v3], [0, 1, 2, 3])

@@ -147,3 +147,55 @@ def test_fast_interp_exact_for_cubics_and_zero_outside(orc):
     assert np.abs(got - ref).max() <= 1e-13
     out = orc.fast_interp(grid, dof, N, np.array([[1.1, 0.5, 0.5], [0.5, -0.01, 0.5]]))
     assert (out == 0).all()
+
+
+def test_velocity_kinds_reduce_to_the_tree_functor(orc, tb):
+    # the single-tree velocity is the NodeFieldFunctor trajectory itself;
+    # a velocity set over V * (a + b t_k) is V scaled by a + b t (Hermite
+    # interpolation is exact for linear data)
+    vel = rotation_tree(orc, tb, 2, 5, 1.3)
+    x = np.random.default_rng(11).random((2000, 3))
+    g = lambda t: 0.8 - 0.6 * t
+    ref = orc.traj_rk2(vel, x, 0.5, 0.3, 3, 1, g=g)
+    got = orc.traj_rk2_v(orc.velocity(orc.VEL_TREE, [vel], g=g), x, 0.5, 0.3, 3, 1)
+    assert (got == ref).all()
+    tt = [0.25, 0.45, 0.55, 0.65]
+    trees = [orc.Tree(vel.anchor, vel.depth, vel.q, 3, vel.coeff * g(t)) for t in tt]
+    got = orc.traj_rk2_v(orc.velocity(orc.VEL_SET, trees, tt), x, 0.5, 0.3, 3, 1)
+    assert np.abs(got - ref).max() <= 1e-14
+
+
+def test_extrapolating_rk2_closed_form(orc, tb):
+    # traj.inc:70-115: stage 1 takes the current field A1 (x - c), stage 2
+    # the extrapolation A2 = 1.5 A1 - 0.5 A0; for linear fields one step is
+    # x - c -> (I + tau A2 + tau^2/2 A2 A1)(x - c)
+    w1, w0 = 2.0, 1.2
+    cur = rotation_tree(orc, tb, 2, 3, w1)
+    prev = rotation_tree(orc, tb, 2, 3, w0)
+    G = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 0]], dtype=float)
+    A1, A2 = w1 * G, (1.5 * w1 - 0.5 * w0) * G
+    c = np.array([0.5, 0.5, 0.0])
+    x = np.random.default_rng(2).uniform(0.3, 0.7, (500, 3))
+    tau = -0.02
+    got = orc.traj_rk2_v(orc.velocity(orc.VEL_EXTRAP, [cur, prev]), x, 1.0, 1.0 + tau, 1, 0)
+    Mk = np.eye(3) + tau * A2 + 0.5 * tau * tau * A2 @ A1
+    assert np.abs(got - (c + (x - c) @ Mk.T)).max() <= 1e-14
+    # swapping the trees changes the result (the roles are not symmetric)
+    swp = orc.traj_rk2_v(orc.velocity(orc.VEL_EXTRAP, [prev, cur]), x, 1.0, 1.0 + tau, 1, 0)
+    assert np.abs(swp - got).max() > 1e-4
+
+
+def test_insitu_with_velocity_kinds(orc, tb):
+    from tbslas_amd import fields
+
+    q = 5
+    vel = rotation_tree(orc, tb, 2, q, 2 * math.pi)
+    anchor, dep = tb.uniform_leaves(2)
+    cf = tb.fit_function(lambda p: fields.gaussian(p, (0.6, 0.5, 0.5), 0.15), anchor, dep, q, 1)
+    M = orc.pt2coeff(q)
+    a = orc.Tree(anchor, dep, q, 1, cf)
+    b = orc.Tree(anchor, dep, q, 1, cf)
+    orc.semilag_insitu(vel, a, M, 2, 0.01, 2, 0)
+    orc.semilag_insitu_v(orc.velocity(orc.VEL_EXTRAP, [vel, vel]), b, M, 2, 0.01, 2, 0)
+    # 1.5 v - 0.5 v differs from v by rounding only
+    assert np.abs(a.coeff - b.coeff).max() <= 1e-13 * np.abs(a.coeff).max()
