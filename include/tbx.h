@@ -61,6 +61,9 @@ enum { TBX_BC_FREESPACE = 0, TBX_BC_PERIODIC = 1 }; /* sim_config.h:59 bc */
                        /* bit-identical to the CPU path                  */
 #define TBX_FP32 0x2u  /* FP32 coefficients and accumulation; point      */
                        /* coordinates, RK2 updates and keys stay FP64    */
+#define TBX_FP32_HEAD64 0x3u /* TBX_FP32 for the terms of total degree > 2, */
+                             /* FP64 for the 10 terms of degree <= 2, which */
+                             /* carry most of a smooth field's magnitude    */
 #define TBX_MODE_MASK 0xFu
 /* EvalTree wraps the caller's points in place under a periodic bc
  * (tree_functor.h:442-449, const_cast at :803).  tbx_eval does so only when
@@ -228,8 +231,13 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
                   uint8_t *depth, int bc);
 
 /* ---- tuning knobs (process-wide) ---------------------------------------
- * TBX_OPT_STEP_SHAPE: launch shape of the fused step kernel (0..3; see
- * DESIGN notes in the source); the value is used by later step calls. */
+ * TBX_OPT_STEP_SHAPE: launch shape of the fused step kernel, used by later
+ * step calls.  0: 256 threads; 1-3: 256 / 1024 / 512 threads with a
+ * register budget of 4 waves per SIMD; 4-7: the workgroup's points
+ * regrouped by leaf before each evaluation, 256 / 512 threads (budget 2
+ * waves per SIMD) and 1024 / 512 threads (budget 4).  6 is the default;
+ * shapes other than 0 and 6 exist for q = 8 and 14 only (others run 6).
+ * Every shape gives the same results. */
 enum { TBX_OPT_STEP_SHAPE = 1 };
 int tbx_set_option(int option, int value);
 int tbx_get_option(int option, int *value);

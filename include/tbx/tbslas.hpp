@@ -9,7 +9,15 @@
 //   tbslas::ComputeTrajRK2 (traj.inc:48-68)     tbx::ComputeTrajRK2
 //   tbslas::SolveSemilagRK2 (semilag.inc:26-45) tbx::SolveSemilagRK2
 //   tbslas::SolveSemilagInSitu                  tbx::SolveSemilagInSitu
-//     (tree_semilag.h:92-135)
+//     (tree_semilag.h:92-181)
+//   tbslas::FieldExtrapFunctor                  tbx::FieldExtrapFunctor
+//     (tree_extrap_functor.h:27-90)               over tbx_velocity_eval
+//   tbslas::FieldSetFunctor                     tbx::FieldSetFunctor
+//     (tree_set_functor.h:27-95)
+//   the extrapolating overloads of              the same overloads
+//     IntegrateRK2 / ComputeTrajRK2
+//     (traj.inc:70-115), SolveSemilagRK2
+//     (semilag.inc:47-69)
 //
 // The functor keeps the reference's calling convention: host pointers,
 // AoS points, point-major values, and the in-place periodic wrap of the
@@ -180,18 +188,168 @@ class ScaledFieldFunctor {
   void *ctx_;
 };
 
+// Evaluation of a tbx_velocity on host arrays through device staging, the
+// points wrapped in place when periodic (each NodeFieldFunctor the
+// reference's composite functors build does so).
+class VelocityStaging {
+ public:
+  void eval(const tbx_velocity &v, int bc, Arith a, const double *pts, int num_points,
+            double t, double *out) {
+    if (num_points <= 0) return;
+    const size_t n = (size_t)num_points;
+    pts_.resize(3 * n);
+    vals_.resize(3 * n);
+    hip_check(hipMemcpy(pts_.get(), pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice),
+              "hipMemcpy");
+    check(tbx_velocity_eval(&v, pts_.get(), (int64_t)n, t, bc,
+                            (uint32_t)a | TBX_WRITE_BACK_WRAP, vals_.get(), nullptr));
+    hip_check(hipMemcpy(out, vals_.get(), sizeof(double) * 3 * n, hipMemcpyDeviceToHost),
+              "hipMemcpy");
+    if (bc == TBX_BC_PERIODIC)
+      hip_check(hipMemcpy(const_cast<double *>(pts), pts_.get(), sizeof(double) * 3 * n,
+                          hipMemcpyDeviceToHost),
+                "hipMemcpy");
+  }
+
+ private:
+  DeviceBuffer<double> pts_, vals_;
+};
+
+// FieldExtrapFunctor (tree_extrap_functor.h:27-90): 1.5 v(t^n) - 0.5
+// v(t^n-1) from the current and previous dof-3 velocity trees.
+template <typename real_t = double>
+class FieldExtrapFunctor {
+  static_assert(std::is_same<real_t, double>::value, "the engine's coordinates are FP64");
+
+ public:
+  FieldExtrapFunctor(Tree *tp, Tree *tc, Arith a = Arith::kFp64)
+      : tp_(tp), tc_(tc), arith_(a), buf_(std::make_shared<VelocityStaging>()) {}
+  virtual ~FieldExtrapFunctor() {}
+
+  void operator()(const real_t *points_pos, int num_points, real_t *out) {
+    buf_->eval(velocity(), tc_->bc(), arith_, points_pos, num_points, 0.0, out);
+  }
+
+  // Shifts the pair: previous <- current <- new_tree.  The reference deletes
+  // the dropped tree; here trees belong to the caller, so it is returned.
+  Tree *update(Tree *new_tree, real_t /*time*/) {
+    Tree *dropped = tp_;
+    tp_ = tc_;
+    tc_ = new_tree;
+    return dropped;
+  }
+
+  // The device functor pairing NodeFieldFunctor(current) for the first RK
+  // stage with this extrapolation for the second (traj.inc:80, 86).
+  tbx_velocity velocity() const {
+    tbx_velocity v{};
+    v.kind = TBX_VEL_EXTRAP;
+    v.trees[0] = tc_->handle();
+    v.trees[1] = tp_->handle();
+    return v;
+  }
+  Tree *current() const { return tc_; }
+  Tree *previous() const { return tp_; }
+  Arith arith() const { return arith_; }
+
+ private:
+  Tree *tp_, *tc_;
+  Arith arith_;
+  std::shared_ptr<VelocityStaging> buf_;
+};
+
+// FieldSetFunctor (tree_set_functor.h:27-95): four dof-3 trees at four
+// times, cubic Hermite interpolation in time (utils/cubic.h:42-56).
+template <typename real_t = double>
+class FieldSetFunctor {
+  static_assert(std::is_same<real_t, double>::value, "the engine's coordinates are FP64");
+
+ public:
+  FieldSetFunctor(std::vector<Tree *> field_set_elems, std::vector<real_t> field_set_times,
+                  Arith a = Arith::kFp64)
+      : elems_(std::move(field_set_elems)), times_(std::move(field_set_times)), arith_(a),
+        buf_(std::make_shared<VelocityStaging>()) {
+    if (elems_.size() != 4 || times_.size() != 4)
+      throw Error(TBX_EINVAL, "FieldSetFunctor takes four trees and four times");
+  }
+  virtual ~FieldSetFunctor() {}
+
+  void operator()(const real_t *points_pos, int num_points, real_t query_time, real_t *out) {
+    buf_->eval(velocity(), elems_[0]->bc(), arith_, points_pos, num_points, query_time, out);
+  }
+
+  // Pops the oldest tree and time, appends the new ones; the dropped tree
+  // is returned to its owner (the reference deletes it).
+  Tree *update(Tree *new_tree, real_t time) {
+    Tree *dropped = elems_.front();
+    elems_.erase(elems_.begin());
+    times_.erase(times_.begin());
+    elems_.push_back(new_tree);
+    times_.push_back(time);
+    return dropped;
+  }
+
+  tbx_velocity velocity() const {
+    tbx_velocity v{};
+    v.kind = TBX_VEL_SET;
+    for (int k = 0; k < 4; ++k) {
+      v.trees[k] = elems_[k]->handle();
+      v.times[k] = times_[k];
+    }
+    return v;
+  }
+  Tree *tree() const { return elems_[0]; }
+  Arith arith() const { return arith_; }
+
+ private:
+  std::vector<Tree *> elems_;
+  std::vector<real_t> times_;
+  Arith arith_;
+  std::shared_ptr<VelocityStaging> buf_;
+};
+
 namespace detail {
+// Functors that are device velocity sources, as a tbx_velocity plus the
+// boundary condition and arithmetic they evaluate with.
 template <typename F>
 struct device_field : std::false_type {};
 template <typename R>
 struct device_field<NodeFieldFunctor<R>> : std::true_type {
-  static tbx_tfac_fn g(const NodeFieldFunctor<R> &) { return nullptr; }
-  static void *ctx(const NodeFieldFunctor<R> &) { return nullptr; }
+  static tbx_velocity velocity(const NodeFieldFunctor<R> &f) {
+    tbx_velocity v{};
+    v.kind = TBX_VEL_TREE;
+    v.trees[0] = f.tree()->handle();
+    return v;
+  }
+  static int bc(const NodeFieldFunctor<R> &f) { return f.tree()->bc(); }
 };
 template <typename R>
 struct device_field<ScaledFieldFunctor<R>> : std::true_type {
-  static tbx_tfac_fn g(const ScaledFieldFunctor<R> &f) { return f.g(); }
-  static void *ctx(const ScaledFieldFunctor<R> &f) { return f.ctx(); }
+  static tbx_velocity velocity(const ScaledFieldFunctor<R> &f) {
+    tbx_velocity v{};
+    v.kind = TBX_VEL_TREE;
+    v.trees[0] = f.tree()->handle();
+    v.g = f.g();
+    v.gctx = f.ctx();
+    return v;
+  }
+  static int bc(const ScaledFieldFunctor<R> &f) { return f.tree()->bc(); }
+};
+template <typename R>
+struct device_field<FieldSetFunctor<R>> : std::true_type {
+  static tbx_velocity velocity(const FieldSetFunctor<R> &f) { return f.velocity(); }
+  static int bc(const FieldSetFunctor<R> &f) { return f.tree()->bc(); }
+};
+
+// (field_fn, extrap_fn) pairs that one device velocity covers:
+// NodeFieldFunctor(current) with FieldExtrapFunctor(previous, current).
+template <typename F, typename E>
+struct device_extrap_pair : std::false_type {};
+template <typename R>
+struct device_extrap_pair<NodeFieldFunctor<R>, FieldExtrapFunctor<R>> : std::true_type {
+  static bool covers(const NodeFieldFunctor<R> &f, const FieldExtrapFunctor<R> &e) {
+    return f.tree() == e.current() && f.arith() == e.arith();
+  }
 };
 }  // namespace detail
 
@@ -220,10 +378,10 @@ void ComputeTrajRK2(Functor field_fn, const std::vector<real_t> &xinit, const re
     DeviceBuffer<double> x(3 * n);
     hip_check(hipMemcpy(x.get(), xinit.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice),
               "hipMemcpy");
-    check(tbx_traj_rk2(field_fn.tree()->handle(), detail::device_field<Functor>::g(field_fn),
-                       detail::device_field<Functor>::ctx(field_fn), x.get(), (int64_t)n, tinit,
-                       tfinal, num_rk_step, field_fn.tree()->bc(), (uint32_t)field_fn.arith(),
-                       x.get(), nullptr));
+    const tbx_velocity v = detail::device_field<Functor>::velocity(field_fn);
+    check(tbx_traj_rk2_v(&v, x.get(), (int64_t)n, tinit, tfinal, num_rk_step,
+                         detail::device_field<Functor>::bc(field_fn), (uint32_t)field_fn.arith(),
+                         x.get(), nullptr));
     hip_check(hipMemcpy(xsol.data(), x.get(), sizeof(double) * 3 * n, hipMemcpyDeviceToHost),
               "hipMemcpy");
   } else {
@@ -234,6 +392,53 @@ void ComputeTrajRK2(Functor field_fn, const std::vector<real_t> &xinit, const re
       IntegrateRK2(field_fn, xsol, t, tau, xsol);
       t = t + tau;
     }
+  }
+}
+
+// IntegrateRK2 with a separate functor for the second stage (traj.inc:70-95):
+// field_fn at xinit, extrap_fn at the midpoint, both without a time.
+template <typename real_t, typename FieldFunctor, typename ExtrapFunctor>
+void IntegrateRK2(FieldFunctor field_fn, ExtrapFunctor extrap_fn,
+                  const std::vector<real_t> &xinit, const real_t /*tinit*/, const real_t dt,
+                  std::vector<real_t> &xsol) {
+  const int num_points = (int)(xinit.size() / 3);
+  std::vector<real_t> xtmp(xinit.size()), vtmp(xinit.size());
+  field_fn(xinit.data(), num_points, vtmp.data());
+  for (size_t i = 0; i < xinit.size(); ++i) xtmp[i] = xinit[i] + 0.5 * dt * vtmp[i];
+  extrap_fn(xtmp.data(), num_points, vtmp.data());
+  for (size_t i = 0; i < xinit.size(); ++i) xsol[i] = xinit[i] + dt * vtmp[i];
+}
+
+// ComputeTrajRK2 with the extrapolating second stage (traj.inc:97-115).
+// NodeFieldFunctor(current) + FieldExtrapFunctor(previous, current), the
+// pairing the reference's drivers use (advtvextrap.cpp:294-300), runs as
+// one device call; other pairs run the host loop over their functors.
+template <typename real_t, typename FieldFunctor, typename ExtrapFunctor>
+void ComputeTrajRK2(FieldFunctor field_fn, ExtrapFunctor extrap_fn,
+                    const std::vector<real_t> &xinit, const real_t tinit, const real_t tfinal,
+                    const int num_rk_step, std::vector<real_t> &xsol) {
+  if (xsol.size() != xinit.size()) xsol.resize(xinit.size());
+  if constexpr (detail::device_extrap_pair<FieldFunctor, ExtrapFunctor>::value) {
+    if (detail::device_extrap_pair<FieldFunctor, ExtrapFunctor>::covers(field_fn, extrap_fn)) {
+      const size_t n = xinit.size() / 3;
+      if (n == 0) return;
+      DeviceBuffer<double> x(3 * n);
+      hip_check(hipMemcpy(x.get(), xinit.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice),
+                "hipMemcpy");
+      const tbx_velocity v = extrap_fn.velocity();
+      check(tbx_traj_rk2_v(&v, x.get(), (int64_t)n, tinit, tfinal, num_rk_step,
+                           field_fn.tree()->bc(), (uint32_t)field_fn.arith(), x.get(), nullptr));
+      hip_check(hipMemcpy(xsol.data(), x.get(), sizeof(double) * 3 * n, hipMemcpyDeviceToHost),
+                "hipMemcpy");
+      return;
+    }
+  }
+  const real_t tau = (tfinal - tinit) / num_rk_step;
+  for (size_t i = 0; i < xinit.size(); ++i) xsol[i] = xinit[i];
+  real_t t = tinit;
+  for (int s = 0; s < num_rk_step; ++s) {
+    IntegrateRK2(field_fn, extrap_fn, xsol, t, tau, xsol);
+    t = t + tau;
   }
 }
 
@@ -250,6 +455,22 @@ void SolveSemilagRK2(VFieldFunctor vel_evaluator, CFieldFunctor con_evaluator,
   con_evaluator(departure.data(), (int)num_points, points_vals.data());
 }
 
+// SolveSemilagRK2 with the extrapolating second stage (semilag.inc:47-69).
+template <typename real_t, typename VFieldFunctor, typename VExtrapFunctor,
+          typename CFieldFunctor>
+void SolveSemilagRK2(VFieldFunctor vel_evaluator, VExtrapFunctor vel_extrap_evaluator,
+                     CFieldFunctor con_evaluator, const std::vector<real_t> &points_pos,
+                     const int sdim, const int timestep, const real_t dt, const int num_rk_step,
+                     std::vector<real_t> &points_vals) {
+  const size_t num_points = points_pos.size() / sdim;
+  const real_t tinit = timestep * dt;
+  const real_t tfinal = tinit - dt;
+  std::vector<real_t> departure(points_pos.size());
+  ComputeTrajRK2(vel_evaluator, vel_extrap_evaluator, points_pos, tinit, tfinal, num_rk_step,
+                 departure);
+  con_evaluator(departure.data(), (int)num_points, points_vals.data());
+}
+
 // SolveSemilagInSitu (tree_semilag.h:92-135) on a device concentration
 // tree: one fused device step (arrival points, RK2, evaluation, refit).
 template <typename VelFunctor>
@@ -258,10 +479,27 @@ void SolveSemilagInSitu(VelFunctor &tvel_func, Tree &tree_curr, const int timest
                         Arith a = Arith::kFp64) {
   static_assert(detail::device_field<VelFunctor>::value,
                 "SolveSemilagInSitu needs a device velocity functor");
-  check(tbx_semilag_insitu(tvel_func.tree()->handle(), detail::device_field<VelFunctor>::g(tvel_func),
-                           detail::device_field<VelFunctor>::ctx(tvel_func), tree_curr.handle(),
-                           timestep, dt, num_rk_step, tree_curr.bc(), (uint32_t)a, nullptr,
-                           nullptr));
+  const tbx_velocity v = detail::device_field<VelFunctor>::velocity(tvel_func);
+  check(tbx_semilag_insitu_v(&v, tree_curr.handle(), timestep, dt, num_rk_step, tree_curr.bc(),
+                             (uint32_t)a, nullptr, nullptr));
+}
+
+// SolveSemilagInSitu with the extrapolating second stage
+// (tree_semilag.h:137-181): NodeFieldFunctor(current) and
+// FieldExtrapFunctor(previous, current).
+template <typename R>
+void SolveSemilagInSitu(NodeFieldFunctor<R> &tvel_func, FieldExtrapFunctor<R> &tvel_extrap,
+                        Tree &tree_curr, const int timestep, const double dt,
+                        int num_rk_step = 1, bool /*adaptive*/ = true,
+                        Arith a = Arith::kFp64) {
+  if (!detail::device_extrap_pair<NodeFieldFunctor<R>, FieldExtrapFunctor<R>>::covers(
+          tvel_func, tvel_extrap))
+    throw Error(TBX_EUNSUPPORTED,
+                "SolveSemilagInSitu: the field functor must evaluate the extrapolation's "
+                "current tree");
+  const tbx_velocity v = tvel_extrap.velocity();
+  check(tbx_semilag_insitu_v(&v, tree_curr.handle(), timestep, dt, num_rk_step, tree_curr.bc(),
+                             (uint32_t)a, nullptr, nullptr));
 }
 
 }  // namespace tbx

@@ -17,6 +17,7 @@ BC_PERIODIC = 1
 FP64 = 0x0
 EXACT = 0x1
 FP32 = 0x2
+FP32_HEAD64 = 0x3
 WRITE_BACK_WRAP = 0x10
 MAX_Q = 16
 MAX_DEPTH = 21
@@ -25,7 +26,7 @@ VEL_TREE = 0
 VEL_EXTRAP = 1
 VEL_SET = 2
 
-MODES = {"fp64": FP64, "exact": EXACT, "fp32": FP32}
+MODES = {"fp64": FP64, "exact": EXACT, "fp32": FP32, "fp32h": FP32_HEAD64}
 
 TFAC_FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
 

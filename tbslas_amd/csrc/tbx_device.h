@@ -27,6 +27,10 @@ template <>
 struct ModeType<kFp32> {
   using T = float;
 };
+template <>
+struct ModeType<kFp32h> {
+  using T = float;
+};
 
 // acc + a*b.  kExact64 keeps the reference's separate multiply and add
 // (vec_eval: add_intrin(w, mul_intrin(px, c)), tree_functor.h:63-76);
@@ -220,7 +224,8 @@ __device__ __forceinline__ T contract(const T *c, const T *px, const T *py,
         }
         const int r = idx % BN;
         const T ck = r < CN ? cur0[r] : cur1[r - CN];
-        w = madd<MODE>(px[k], ck, w);
+        if (MODE != kFp32h || i + j + k > kHeadDeg)  // kFp32h: head in FP64
+          w = madd<MODE>(px[k], ck, w);
         ++idx;
       }
       v = madd<MODE>(py[j], w, v);
@@ -238,7 +243,7 @@ constexpr int ncoef() {
 template <int MODE>
 __device__ __forceinline__ const typename ModeType<MODE>::T *coeff_base(
     const TreeView &t) {
-  if constexpr (MODE == kFp32)
+  if constexpr (MODE == kFp32 || MODE == kFp32h)
     return t.coeff32;
   else
     return t.coeff;
@@ -276,35 +281,89 @@ struct PointBasis {
   using T = typename ModeType<MODE>::T;
   int leaf;
   T px[D], py[D], pz[D];
+  double hx[3], hy[3], hz[3];  // kFp32h: T_0..T_2 in FP64
 
+  // hint: the point's leaf when the caller has located it already, else -1
   __device__ __forceinline__ void build(const TreeView &t, double x, double y,
-                                        double z) {
-    leaf = locate(t, x, y, z);
+                                        double z, int hint = -1) {
+    leaf = hint >= 0 ? hint : locate(t, x, y, z);
     double ex, ey, ez;
     leaf_xi(t, leaf, x, y, z, ex, ey, ez);
     cheb_poly<D>(ex, px);
     cheb_poly<D>(ey, py);
     cheb_poly<D>(ez, pz);
+    if constexpr (MODE == kFp32h) {
+      cheb_poly<3>(ex, hx);
+      cheb_poly<3>(ey, hy);
+      cheb_poly<3>(ez, hz);
+    }
   }
 };
+
+// Packed index of coefficient (i, j, k) in a degree D - 1 triangle: the
+// i(z) -> j(y) -> k(x) order of vec_eval (tree_functor.h:258-266).
+__host__ __device__ constexpr int tri_index(int D, int i, int j, int k) {
+  int idx = 0;
+  for (int a = 0; a < i; ++a) idx += (D - a) * (D - a + 1) / 2;
+  for (int b = 0; b < j; ++b) idx += D - i - b;
+  return idx + k;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T *p) {
+  return *(const __attribute__((address_space(4))) T *)(p);
+}
+
+// The head of one leaf's contraction in FP64: the terms with
+// i + j + k <= kHeadDeg, nested as the full contraction (k, then j, then i),
+// from the FP64 coefficients c (wave-uniform: scalar loads).
+template <int D>
+__device__ __forceinline__ double head_contract(const double *c, const double *hx,
+                                                const double *hy, const double *hz) {
+  constexpr int H = kHeadDeg < D - 1 ? kHeadDeg : D - 1;
+  double u = 0.0;
+#pragma unroll
+  for (int i = 0; i <= H; ++i) {
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; i + j <= H; ++j) {
+      double w = 0.0;
+#pragma unroll
+      for (int k = 0; i + j + k <= H; ++k)
+        w = __builtin_fma(hx[k], load_uniform(c + tri_index(D, i, j, k)), w);
+      v = __builtin_fma(hy[j], w, v);
+    }
+    u = __builtin_fma(hz[i], v, u);
+  }
+  return u;
+}
+
+// One component of one leaf at the lane's point, in the mode's arithmetic.
+template <int D, int MODE>
+__device__ __forceinline__ double leaf_value(const TreeView &t, int lf, int comp,
+                                             const PointBasis<D, MODE> &b) {
+  constexpr int NC = D * (D + 1) * (D + 2) / 6;
+  const size_t off = ((size_t)lf * t.dof + comp) * NC;
+  const double u = (double)contract<D, MODE>(coeff_base<MODE>(t) + off, b.px, b.py, b.pz);
+  if constexpr (MODE == kFp32h)
+    return u + head_contract<D>(t.coeff + off, b.hx, b.hy, b.hz);
+  else
+    return u;
+}
 
 // Velocity (dof 3) at one point: the waterfall over the wave's leaves.
 template <int D, int MODE>
 __device__ __forceinline__ void eval_vec3(const TreeView &t, double x,
                                           double y, double z, double &v0,
-                                          double &v1, double &v2) {
-  using T = typename ModeType<MODE>::T;
-  constexpr int NC = ncoef<D>();
+                                          double &v1, double &v2, int hint = -1) {
   PointBasis<D, MODE> b;
-  b.build(t, x, y, z);
-  const T *base = coeff_base<MODE>(t);
+  b.build(t, x, y, z, hint);
   for_each_wave_leaf(b.leaf, [&](int lf) {
-    const T *c = base + (size_t)lf * (3 * NC);
     // one component at a time: unrolled side by side, the three
     // contractions' scalar loads exhaust the SGPRs and spill
 #pragma unroll 1
     for (int l = 0; l < 3; ++l) {
-      const double u = (double)contract<D, MODE>(c + l * NC, b.px, b.py, b.pz);
+      const double u = leaf_value<D, MODE>(t, lf, l, b);
       v0 = l == 0 ? u : v0;
       v1 = l == 1 ? u : v1;
       v2 = l == 2 ? u : v2;
@@ -315,17 +374,12 @@ __device__ __forceinline__ void eval_vec3(const TreeView &t, double x,
 // All dof components at one point, each written through store(l, value).
 template <int D, int MODE, typename Store>
 __device__ __forceinline__ void eval_dof(const TreeView &t, double x, double y,
-                                         double z, Store store) {
-  using T = typename ModeType<MODE>::T;
-  constexpr int NC = ncoef<D>();
+                                         double z, Store store, int hint = -1) {
   PointBasis<D, MODE> b;
-  b.build(t, x, y, z);
-  const T *base = coeff_base<MODE>(t);
+  b.build(t, x, y, z, hint);
   const int dof = t.dof;
   for_each_wave_leaf(b.leaf, [&](int lf) {
-    const T *c = base + (size_t)lf * dof * NC;
-    for (int l = 0; l < dof; ++l)
-      store(l, (double)contract<D, MODE>(c + (size_t)l * NC, b.px, b.py, b.pz));
+    for (int l = 0; l < dof; ++l) store(l, leaf_value<D, MODE>(t, lf, l, b));
   });
 }
 
@@ -349,12 +403,19 @@ __device__ __forceinline__ double cubic1d(double x, const double *xx, double p0,
 // rolled loop (one contraction body) and are combined as the reference's
 // FieldExtrapFunctor (1.5 c - 0.5 p, tree_extrap_functor.h:71-77) or
 // FieldSetFunctor (cubic Hermite in time, tree_set_functor.h:66-72).
+// Those functors evaluate their trees one NodeFieldFunctor after another on
+// the same array, and each EvalTree wraps that array in place
+// (tree_functor.h:442-449): the point is wrapped again before every tree
+// after the first (the caller wrapped it for the first), and the caller's
+// point ends up wrapped once per tree.  This matters only for points more
+// than one period outside the domain.
 template <int D, int MODE>
 __device__ __forceinline__ void eval_velocity(const VelSource &vs, int stage, double ts,
-                                              int scaled, double g, double x, double y,
-                                              double z, double &v0, double &v1, double &v2) {
+                                              int scaled, double g, int bc, double &x,
+                                              double &y, double &z, double &v0, double &v1,
+                                              double &v2, int hint0 = -1) {
   if (vs.kind == kVelTree) {
-    eval_vec3<D, MODE>(vs.t[0], x, y, z, v0, v1, v2);
+    eval_vec3<D, MODE>(vs.t[0], x, y, z, v0, v1, v2, hint0);
     if (scaled) {
       v0 = v0 * g;
       v1 = v1 * g;
@@ -367,9 +428,14 @@ __device__ __forceinline__ void eval_velocity(const VelSource &vs, int stage, do
   double c0 = 0, c1 = 0, c2 = 0, d0 = 0, d1 = 0, d2 = 0;
 #pragma unroll 1
   for (int k = 0; k < nt; ++k) {
+    if (k > 0 && bc == TBX_BC_PERIODIC) {
+      x = wrap1(x);
+      y = wrap1(y);
+      z = wrap1(z);
+    }
     const TreeView &tk = k == 0 ? vs.t[0] : k == 1 ? vs.t[1] : k == 2 ? vs.t[2] : vs.t[3];
     double r0, r1, r2;
-    eval_vec3<D, MODE>(tk, x, y, z, r0, r1, r2);
+    eval_vec3<D, MODE>(tk, x, y, z, r0, r1, r2, k == 0 ? hint0 : -1);
     if (k == 0) {
       a0 = r0; a1 = r1; a2 = r2;
     } else if (k == 1) {
@@ -412,7 +478,7 @@ __device__ __forceinline__ void rk2_substep(const VelSource &vel, int bc,
     z = wrap1(z);
   }
   double v0, v1, v2;
-  eval_velocity<D, MODE>(vel, 1, ts1, scaled, g1, x, y, z, v0, v1, v2);
+  eval_velocity<D, MODE>(vel, 1, ts1, scaled, g1, bc, x, y, z, v0, v1, v2);
   // xtmp = xinit + (0.5 * dt) * vtmp   (traj.inc:36)
   double hx = x + htau * v0, hy = y + htau * v1, hz = z + htau * v2;
   if (bc == TBX_BC_PERIODIC) {
@@ -420,11 +486,92 @@ __device__ __forceinline__ void rk2_substep(const VelSource &vel, int bc,
     hy = wrap1(hy);
     hz = wrap1(hz);
   }
-  eval_velocity<D, MODE>(vel, 2, ts2, scaled, g2, hx, hy, hz, v0, v1, v2);
+  eval_velocity<D, MODE>(vel, 2, ts2, scaled, g2, bc, hx, hy, hz, v0, v1, v2);
   // xsol = xinit + dt * vtmp   (traj.inc:41-42)
   x = x + tau * v0;
   y = y + tau * v1;
   z = z + tau * v2;
+}
+
+// ---------------------------------------------------------------------------
+// Regrouping a workgroup's points by leaf.
+//
+// Consecutive arrival points lie mostly in one leaf, but the grid points on
+// a leaf's upper faces belong to the neighbours (half-open boxes) and the
+// departure points near any face scatter across it, so a wavefront of 64
+// consecutive points touches about 2.7 distinct leaves and the waterfall
+// repeats the contraction that often.  Before each evaluation the
+// workgroup therefore permutes its points through LDS so that the points of
+// one leaf are contiguous: leaves get slots in a small open-addressing hash
+// table, slots get counts and offsets, and each point moves to offset +
+// rank.  A wavefront then spans one leaf except where two slots meet.  The
+// order inside a slot is arbitrary; results do not depend on it, because
+// each point's arithmetic is independent of the lane that does it.
+// ---------------------------------------------------------------------------
+constexpr int kRgSlots = 64;  // hash slots; one more collects any overflow
+
+template <int BLOCK, int NF>
+struct RegroupLds {
+  int key[kRgSlots];
+  int cnt[kRgSlots + 1];
+  int orig[BLOCK];
+  int leaf[BLOCK];
+  double f[NF][BLOCK];
+};
+
+// New position of this thread's item, items of equal key contiguous.
+// Every thread of the workgroup must call it (it synchronises the block).
+template <int BLOCK, int NF>
+__device__ __forceinline__ int regroup_pos(RegroupLds<BLOCK, NF> &s, int key) {
+  const int tid = threadIdx.x;
+  if (tid < kRgSlots) s.key[tid] = -1;
+  if (tid <= kRgSlots) s.cnt[tid] = 0;
+  __syncthreads();
+  uint32_t h = ((uint32_t)key * 0x9E3779B1u) >> 26;  // 6 bits: kRgSlots == 64
+  int slot = kRgSlots;
+#pragma unroll 1
+  for (int p = 0; p < kRgSlots; ++p) {
+    const int prev = atomicCAS(&s.key[h], -1, key);
+    if (prev == -1 || prev == key) {
+      slot = (int)h;
+      break;
+    }
+    h = (h + 1) & (kRgSlots - 1);
+  }
+  const int rank = atomicAdd(&s.cnt[slot], 1);
+  __syncthreads();
+  if (tid < 64) {  // exclusive scan of the 64 slot counts by wave 0
+    const int c = s.cnt[tid];
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(incl, d, 64);
+      if (tid >= d) incl += o;
+    }
+    s.cnt[tid] = incl - c;
+    if (tid == 63) s.cnt[kRgSlots] = incl;  // the overflow slot goes last
+  }
+  __syncthreads();
+  return s.cnt[slot] + rank;
+}
+
+// Moves (orig, leaf, f[0..nf)) of every thread to its regrouped position
+// and returns the item now at this thread's position.
+template <int BLOCK, int NF, int NUSE>
+__device__ __forceinline__ void regroup(RegroupLds<BLOCK, NF> &s, int &orig, int &leaf,
+                                        double (&f)[NUSE]) {
+  static_assert(NUSE <= NF, "payload too wide");
+  const int pos = regroup_pos(s, leaf);
+  s.orig[pos] = orig;
+  s.leaf[pos] = leaf;
+#pragma unroll
+  for (int k = 0; k < NUSE; ++k) s.f[k][pos] = f[k];
+  __syncthreads();
+  const int tid = threadIdx.x;
+  orig = s.orig[tid];
+  leaf = s.leaf[tid];
+#pragma unroll
+  for (int k = 0; k < NUSE; ++k) f[k] = s.f[k][tid];
 }
 
 }  // namespace tbx

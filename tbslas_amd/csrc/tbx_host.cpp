@@ -27,7 +27,7 @@ using namespace tbx;
 namespace {
 
 thread_local std::string g_err;
-std::atomic<int> g_step_shape{0};
+std::atomic<int> g_step_shape{kDefaultStepShape};
 
 int fail(int code, const char *fmt, ...) {
   char buf[512];
@@ -295,13 +295,15 @@ int mode_of(uint32_t flags, int *mode) {
     *mode = kExact64;
   else if (m == TBX_FP32)
     *mode = kFp32;
+  else if (m == TBX_FP32_HEAD64)
+    *mode = kFp32h;
   else
     return fail(TBX_EINVAL, "unknown arithmetic mode 0x%x", m);
   return TBX_OK;
 }
 
 int prepare_mode(tbx_tree *t, int mode, hipStream_t s) {
-  if (mode == kFp32) return ensure_f32(t, s);
+  if (mode == kFp32 || mode == kFp32h) return ensure_f32(t, s);
   return TBX_OK;
 }
 
@@ -745,7 +747,7 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
   const int P = conc->P();
   StepArgs a{};
   const int want = g_step_shape.load();
-  a.shape = step_shape_built(want, conc->q) ? want : 0;
+  a.shape = step_shape_built(want, conc->q) ? want : kDefaultStepShape;
   a.npts = conc->L * P;
   a.nblocks = (a.npts + kStepShapeBlock[a.shape] - 1) / kStepShapeBlock[a.shape];
   a.per_xcd = (a.nblocks + 7) / 8;

@@ -14,7 +14,10 @@ namespace tbx {
 constexpr int kKeyBits = TBX_MAX_DEPTH;
 constexpr double kKeyScale = 2097152.0;  // 2^21
 
-enum Mode : int { kFma64 = 0, kExact64 = 1, kFp32 = 2 };
+// kFp32h: FP32 tail of the contraction (total degree > kHeadDeg) plus the
+// head (total degree <= kHeadDeg) in FP64.
+enum Mode : int { kFma64 = 0, kExact64 = 1, kFp32 = 2, kFp32h = 3 };
+constexpr int kHeadDeg = 2;
 
 // Device view of one tree.  All pointers are device memory owned by the
 // tree handle.
@@ -62,12 +65,17 @@ struct SubArgs {
   int bc;
 };
 
-// Launch shapes of the fused step kernel: {workgroup size, minimum waves
-// per SIMD}.  Shape 0 is built for every degree; the others for the degrees
-// the benchmark workloads use (8 and 14); other degrees fall back to 0.
-constexpr int kStepShapes = 4;
-constexpr int kStepShapeBlock[kStepShapes] = {256, 256, 1024, 512};
-inline bool step_shape_built(int shape, int q) { return shape == 0 || q == 8 || q == 14; }
+// Launch shapes of the fused step kernel (workgroup sizes below).  0-3 run
+// the points in arrival order; 4-7 regroup each workgroup's points by leaf
+// before every evaluation.  Shapes 0 and 6 (the default) are built for
+// every degree, the others for the benchmark degrees 8 and 14 only; an
+// unbuilt shape falls back to 6.
+constexpr int kStepShapes = 8;
+constexpr int kStepShapeBlock[kStepShapes] = {256, 256, 1024, 512, 256, 512, 1024, 512};
+constexpr int kDefaultStepShape = 6;
+inline bool step_shape_built(int shape, int q) {
+  return shape == 0 || shape == 6 || q == 8 || q == 14;
+}
 
 struct StepArgs {
   int32_t shape;      // index into kStepShapeBlock

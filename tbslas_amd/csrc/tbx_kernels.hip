@@ -3,6 +3,8 @@
 // launch_{eval,traj,step}_q<q> declared in tbx_internal.h.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "tbx_device.h"
 
 #ifndef TBX_Q
@@ -57,14 +59,14 @@ __global__ __launch_bounds__(kBlock) void k_veval(VelSource vel,
     x = wrap1(x);
     y = wrap1(y);
     z = wrap1(z);
-    if (write_back) {
-      pts[3 * i + 0] = x;
-      pts[3 * i + 1] = y;
-      pts[3 * i + 2] = z;
-    }
   }
   double v0, v1, v2;
-  eval_velocity<D, MODE>(vel, 2, t, scaled, g, x, y, z, v0, v1, v2);
+  eval_velocity<D, MODE>(vel, 2, t, scaled, g, bc, x, y, z, v0, v1, v2);
+  if (bc == TBX_BC_PERIODIC && write_back) {  // wrapped once per tree
+    pts[3 * i + 0] = x;
+    pts[3 * i + 1] = y;
+    pts[3 * i + 2] = z;
+  }
   out[3 * i + 0] = v0;
   out[3 * i + 1] = v1;
   out[3 * i + 2] = v2;
@@ -131,8 +133,160 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
                     [&](int l, double v) { o[(int64_t)l * a.P] = v; });
 }
 
+// The same step with the workgroup's points regrouped by leaf before each
+// of the 2 num_rk_step + 1 evaluations (regroup_pos in tbx_device.h).  The
+// thread that holds a point after a regroup carries it on: the RK2 state
+// (start point, wrapped midpoint) travels with the point through LDS, and
+// the leaf found for the regroup is the one the evaluation uses.  Lanes
+// past the last arrival point carry a copy of it and store nothing.
+template <int MODE, int BLOCK, int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_step_rg(
+    VelSource vel, TreeView conc, StepArgs a, double *__restrict__ vals,
+    double *__restrict__ dep) {
+  __shared__ RegroupLds<BLOCK, 6> sh;
+  const uint32_t b = blockIdx.x;
+  const int64_t lb = (int64_t)(b & 7u) * a.per_xcd + (int64_t)(b >> 3);
+  if (lb >= a.nblocks) return;  // uniform over the workgroup
+  const int64_t i0 = lb * BLOCK;
+  int orig = (int)threadIdx.x;
+  {
+    const int64_t i = i0 + orig < a.npts ? i0 + orig : a.npts - 1;
+    const int64_t leaf = i / a.P;
+    const int m = (int)(i - leaf * a.P);
+    const uint4 inf = conc.info[leaf];
+    const double len = 1.0 / (double)(1u << inf.w);
+    double st[6];
+    st[0] = (double)inf.x * len + len * a.node[m % D];
+    st[1] = (double)inf.y * len + len * a.node[(m / D) % D];
+    st[2] = (double)inf.z * len + len * a.node[m / (D * D)];
+    int lf = 0;
+    for (int s = 0; s < a.nsub; ++s) {
+      // stage 1 at the (wrapped in place) start point (traj.inc:33)
+      if (a.bc == TBX_BC_PERIODIC) {
+        st[0] = wrap1(st[0]);
+        st[1] = wrap1(st[1]);
+        st[2] = wrap1(st[2]);
+      }
+      lf = locate(vel.t[0], st[0], st[1], st[2]);
+      {
+        double p3[3] = {st[0], st[1], st[2]};
+        regroup(sh, orig, lf, p3);
+        st[0] = p3[0];
+        st[1] = p3[1];
+        st[2] = p3[2];
+      }
+      double v0, v1, v2;
+      eval_velocity<D, MODE>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, st[0], st[1],
+                             st[2], v0, v1, v2, lf);
+      // midpoint xtmp = xinit + (0.5 * dt) * vtmp (traj.inc:36), wrapped
+      // for its evaluation only
+      st[3] = st[0] + a.htau * v0;
+      st[4] = st[1] + a.htau * v1;
+      st[5] = st[2] + a.htau * v2;
+      if (a.bc == TBX_BC_PERIODIC) {
+        st[3] = wrap1(st[3]);
+        st[4] = wrap1(st[4]);
+        st[5] = wrap1(st[5]);
+      }
+      lf = locate(vel.t[0], st[3], st[4], st[5]);
+      regroup(sh, orig, lf, st);
+      eval_velocity<D, MODE>(vel, 2, a.ts[2 * s + 1], a.scaled, a.g[2 * s + 1], a.bc, st[3],
+                             st[4], st[5], v0, v1, v2, lf);
+      // xsol = xinit + dt * vtmp (traj.inc:41-42)
+      st[0] = st[0] + a.tau * v0;
+      st[1] = st[1] + a.tau * v1;
+      st[2] = st[2] + a.tau * v2;
+    }
+    // con_evaluator(departure points) wraps them in place (semilag.inc:43)
+    if (a.bc == TBX_BC_PERIODIC) {
+      st[0] = wrap1(st[0]);
+      st[1] = wrap1(st[1]);
+      st[2] = wrap1(st[2]);
+    }
+    lf = locate(conc, st[0], st[1], st[2]);
+    double p3[3] = {st[0], st[1], st[2]};
+    regroup(sh, orig, lf, p3);
+    const int64_t gi = i0 + orig;
+    const bool live = gi < a.npts;
+    if (live && dep) {
+      dep[3 * gi + 0] = p3[0];
+      dep[3 * gi + 1] = p3[1];
+      dep[3 * gi + 2] = p3[2];
+    }
+    const int64_t al = gi / a.P;
+    const int am = (int)(gi - al * a.P);
+    double *o = vals + al * conc.dof * a.P + am;
+    eval_dof<D, MODE>(
+        conc, p3[0], p3[1], p3[2],
+        [&](int l, double v) {
+          if (live) o[(int64_t)l * a.P] = v;
+        },
+        lf);
+  }
+}
+
 inline unsigned grid_for(int64_t n) {
   return (unsigned)((n + kBlock - 1) / kBlock);
+}
+
+// Calls f with the arithmetic mode as a compile-time constant.
+template <typename F>
+hipError_t with_mode(int mode, F f) {
+  switch (mode) {
+    case kFma64:
+      return f(std::integral_constant<int, kFma64>{});
+    case kExact64:
+      return f(std::integral_constant<int, kExact64>{});
+    case kFp32:
+      return f(std::integral_constant<int, kFp32>{});
+    case kFp32h:
+      return f(std::integral_constant<int, kFp32h>{});
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <int MODE, int BLOCK, int WPE>
+hipError_t launch_step_shape(const VelSource &vel, const TreeView &conc, const StepArgs &a,
+                             double *vals, double *dep, hipStream_t s) {
+  hipLaunchKernelGGL((k_step<MODE, BLOCK, WPE>), dim3((unsigned)(8 * a.per_xcd)), dim3(BLOCK), 0,
+                     s, vel, conc, a, vals, dep);
+  return hipGetLastError();
+}
+
+template <int MODE, int BLOCK, int WPE>
+hipError_t launch_step_rg_shape(const VelSource &vel, const TreeView &conc, const StepArgs &a,
+                                double *vals, double *dep, hipStream_t s) {
+  hipLaunchKernelGGL((k_step_rg<MODE, BLOCK, WPE>), dim3((unsigned)(8 * a.per_xcd)), dim3(BLOCK),
+                     0, s, vel, conc, a, vals, dep);
+  return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_step_mode(const VelSource &vel, const TreeView &conc, const StepArgs &a,
+                            double *vals, double *dep, hipStream_t s) {
+  // launch shapes of kStepShapes (tbx_internal.h); the extra shapes are
+  // built for the degrees of the benchmark workloads only
+  switch (a.shape) {
+    case 0:
+      return launch_step_shape<MODE, 256, 1>(vel, conc, a, vals, dep, s);
+#if TBX_Q == 8 || TBX_Q == 14
+    case 4:
+      return launch_step_rg_shape<MODE, 256, 2>(vel, conc, a, vals, dep, s);
+    case 1:
+      return launch_step_shape<MODE, 256, 4>(vel, conc, a, vals, dep, s);
+    case 2:
+      return launch_step_shape<MODE, 1024, 4>(vel, conc, a, vals, dep, s);
+    case 3:
+      return launch_step_shape<MODE, 512, 4>(vel, conc, a, vals, dep, s);
+    case 5:
+      return launch_step_rg_shape<MODE, 512, 2>(vel, conc, a, vals, dep, s);
+    case 7:
+      return launch_step_rg_shape<MODE, 512, 4>(vel, conc, a, vals, dep, s);
+#endif
+    default:  // shape 6, and the shapes not built for this degree
+      return launch_step_rg_shape<MODE, 1024, 4>(vel, conc, a, vals, dep, s);
+  }
 }
 
 }  // namespace
@@ -145,118 +299,45 @@ hipError_t TBX_CAT(launch_eval_q, TBX_Q)(int mode, const TreeView &t,
                                          int write_back, double *out,
                                          hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const dim3 g(grid_for(n)), blk(kBlock);
-  switch (mode) {
-    case kFma64:
-      hipLaunchKernelGGL(k_eval<kFma64>, g, blk, 0, s, t, pts, n, bc,
-                         write_back, out);
-      break;
-    case kExact64:
-      hipLaunchKernelGGL(k_eval<kExact64>, g, blk, 0, s, t, pts, n, bc,
-                         write_back, out);
-      break;
-    case kFp32:
-      hipLaunchKernelGGL(k_eval<kFp32>, g, blk, 0, s, t, pts, n, bc,
-                         write_back, out);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return with_mode(mode, [&](auto m) {
+    hipLaunchKernelGGL(k_eval<decltype(m)::value>, dim3(grid_for(n)), dim3(kBlock), 0, s, t, pts,
+                       n, bc, write_back, out);
+    return hipGetLastError();
+  });
 }
 
 hipError_t TBX_CAT(launch_veval_q, TBX_Q)(int mode, const VelSource &vel,
                                           double *pts, int64_t n, int bc,
                                           int write_back, double t, int scaled,
-                                          double g, double *out, hipStream_t s) {
+                                          double g, double *out,
+                                          hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const dim3 gr(grid_for(n)), blk(kBlock);
-  switch (mode) {
-    case kFma64:
-      hipLaunchKernelGGL(k_veval<kFma64>, gr, blk, 0, s, vel, pts, n, bc, write_back, t,
-                         scaled, g, out);
-      break;
-    case kExact64:
-      hipLaunchKernelGGL(k_veval<kExact64>, gr, blk, 0, s, vel, pts, n, bc, write_back, t,
-                         scaled, g, out);
-      break;
-    case kFp32:
-      hipLaunchKernelGGL(k_veval<kFp32>, gr, blk, 0, s, vel, pts, n, bc, write_back, t,
-                         scaled, g, out);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return with_mode(mode, [&](auto m) {
+    hipLaunchKernelGGL(k_veval<decltype(m)::value>, dim3(grid_for(n)), dim3(kBlock), 0, s, vel,
+                       pts, n, bc, write_back, t, scaled, g, out);
+    return hipGetLastError();
+  });
 }
 
 hipError_t TBX_CAT(launch_traj_q, TBX_Q)(int mode, const VelSource &vel,
                                          double *x, int64_t n,
                                          const SubArgs &a, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const dim3 g(grid_for(n)), blk(kBlock);
-  switch (mode) {
-    case kFma64:
-      hipLaunchKernelGGL(k_traj<kFma64>, g, blk, 0, s, vel, x, n, a);
-      break;
-    case kExact64:
-      hipLaunchKernelGGL(k_traj<kExact64>, g, blk, 0, s, vel, x, n, a);
-      break;
-    case kFp32:
-      hipLaunchKernelGGL(k_traj<kFp32>, g, blk, 0, s, vel, x, n, a);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return with_mode(mode, [&](auto m) {
+    hipLaunchKernelGGL(k_traj<decltype(m)::value>, dim3(grid_for(n)), dim3(kBlock), 0, s, vel,
+                       x, n, a);
+    return hipGetLastError();
+  });
 }
-
-namespace {
-
-template <int MODE, int BLOCK, int WPE>
-hipError_t launch_step_shape(const VelSource &vel, const TreeView &conc, const StepArgs &a,
-                             double *vals, double *dep, hipStream_t s) {
-  hipLaunchKernelGGL((k_step<MODE, BLOCK, WPE>), dim3((unsigned)(8 * a.per_xcd)), dim3(BLOCK), 0,
-                     s, vel, conc, a, vals, dep);
-  return hipGetLastError();
-}
-
-template <int MODE>
-hipError_t launch_step_mode(const VelSource &vel, const TreeView &conc, const StepArgs &a,
-                            double *vals, double *dep, hipStream_t s) {
-  // launch shapes of kStepShapes (tbx_internal.h); the extra shapes are
-  // built for the degrees of the benchmark workloads only
-  switch (a.shape) {
-#if TBX_Q == 8 || TBX_Q == 14
-    case 1:
-      return launch_step_shape<MODE, 256, 4>(vel, conc, a, vals, dep, s);
-    case 2:
-      return launch_step_shape<MODE, 1024, 4>(vel, conc, a, vals, dep, s);
-    case 3:
-      return launch_step_shape<MODE, 512, 4>(vel, conc, a, vals, dep, s);
-#endif
-    default:
-      return launch_step_shape<MODE, 256, 1>(vel, conc, a, vals, dep, s);
-  }
-}
-
-}  // namespace
 
 hipError_t TBX_CAT(launch_step_q, TBX_Q)(int mode, const VelSource &vel,
                                          const TreeView &conc,
                                          const StepArgs &a, double *vals,
                                          double *dep, hipStream_t s) {
   if (a.npts <= 0) return hipSuccess;
-  switch (mode) {
-    case kFma64:
-      return launch_step_mode<kFma64>(vel, conc, a, vals, dep, s);
-    case kExact64:
-      return launch_step_mode<kExact64>(vel, conc, a, vals, dep, s);
-    case kFp32:
-      return launch_step_mode<kFp32>(vel, conc, a, vals, dep, s);
-    default:
-      return hipErrorInvalidValue;
-  }
+  return with_mode(mode, [&](auto m) {
+    return launch_step_mode<decltype(m)::value>(vel, conc, a, vals, dep, s);
+  });
 }
 
 }  // namespace tbx

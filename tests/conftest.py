@@ -74,3 +74,11 @@ def edge_points():
     pts += [[v, v, v] for v in vals]
     pts += [list(rng.choice(vals, 3)) for _ in range(300)]
     return np.array(pts, dtype=np.float64)
+
+
+def wrap_periodic(x):
+    """The reference's periodic wrap, two sequential tests
+    (tree_functor.h:445-446): c < 0 -> c + 1, then c >= 1 -> c - 1, so a
+    tiny negative c whose c + 1 rounds to 1.0 ends at 0.0."""
+    x = np.where(x < 0, x + 1, x)
+    return np.where(x >= 1, x - 1, x)

@@ -2,7 +2,8 @@
 // (include/tbx/tbslas.hpp), as a tbslas caller would: the rotating-Gaussian
 // case of advection.cpp (test 1) at small size.
 //
-// Prints one line "traj_diff=<a> rk2_diff=<b> linf=<c> sum_drift=<d>":
+// Prints one line "traj_diff=<a> rk2_diff=<b> linf=<c> sum_drift=<d>
+// extrap_diff=<e> set_diff=<f> insitu_extrap_diff=<g>":
 //   a: fused device ComputeTrajRK2 vs the reference's host loop over the
 //      same device functor (both TBX_EXACT: expected 0)
 //   b: SolveSemilagRK2 with device functors vs an explicit traj + eval
@@ -11,6 +12,11 @@
 //      steps against the analytically rotated blob
 //   d: relative change of the plain sum of the grid-point values (moves with
 //      the blob across the grid; semi-Lagrangian steps do not conserve it)
+//   e: ComputeTrajRK2(NodeFieldFunctor, FieldExtrapFunctor) fused vs the
+//      reference's host loop over the same functors (expected 0)
+//   f: ComputeTrajRK2(FieldSetFunctor) fused vs the host loop (expected 0)
+//   g: SolveSemilagInSitu with FieldExtrapFunctor(v, v) vs the plain step
+//      (1.5 v - 0.5 v: rounding only)
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -47,6 +53,19 @@ struct HostLoop {
   void operator()(const double *p, int n, double t, double *o) { (*f)(p, n, t, o); }
   void operator()(const double *p, int n, double *o) { (*f)(p, n, o); }
 };
+
+template <class F>
+struct HostLoopOf {
+  F *f;
+  void operator()(const double *p, int n, double t, double *o) { (*f)(p, n, t, o); }
+  void operator()(const double *p, int n, double *o) { (*f)(p, n, o); }
+};
+
+double max_diff(const std::vector<double> &a, const std::vector<double> &b) {
+  double d = 0;
+  for (size_t i = 0; i < a.size(); ++i) d = std::fmax(d, std::fabs(a[i] - b[i]));
+  return d;
+}
 
 }  // namespace
 
@@ -105,7 +124,32 @@ int main() {
     m0 += before[i];
     m1 += after[i];
   }
-  std::printf("traj_diff=%.3e rk2_diff=%.3e linf=%.3e sum_drift=%.3e\n", traj_diff, rk2_diff,
-              err / mx, std::fabs(m1 - m0) / std::fabs(m0));
+
+  // time-dependent velocity functors: a previous field at 0.8 of the speed
+  std::vector<double> vcf_prev(vcf.size());
+  for (size_t i = 0; i < vcf.size(); ++i) vcf_prev[i] = 0.8 * vcf[i];
+  tbx::Tree velp(anchor, dep, q, 3, vcf_prev);
+  tbx::FieldExtrapFunctor<double> ef(&velp, &vel, tbx::Arith::kExact);
+  tbx::ComputeTrajRK2(vf, ef, x0, 0.3, 0.25, 3, xa);
+  tbx::ComputeTrajRK2(HostLoop{&vf}, HostLoopOf<tbx::FieldExtrapFunctor<double>>{&ef}, x0, 0.3,
+                      0.25, 3, xb);
+  const double extrap_diff = max_diff(xa, xb);
+  tbx::FieldSetFunctor<double> sf({&velp, &vel, &velp, &vel}, {0.2, 0.24, 0.28, 0.32},
+                                  tbx::Arith::kExact);
+  tbx::ComputeTrajRK2(sf, x0, 0.3, 0.25, 3, xa);
+  tbx::ComputeTrajRK2(HostLoopOf<tbx::FieldSetFunctor<double>>{&sf}, x0, 0.3, 0.25, 3, xb);
+  const double set_diff = max_diff(xa, xb);
+  tbx::Tree c1(anchor, dep, q, 1, ccf), c2(anchor, dep, q, 1, ccf);
+  tbx::FieldExtrapFunctor<double> same(&vel, &vel, tbx::Arith::kExact);
+  for (int s = 1; s <= 2; ++s) {
+    tbx::SolveSemilagInSitu(vf, c1, s, dt, 1, true, tbx::Arith::kExact);
+    tbx::SolveSemilagInSitu(vf, same, c2, s, dt, 1, true, tbx::Arith::kExact);
+  }
+  const double insitu_extrap_diff = max_diff(c1.coeffs(), c2.coeffs());
+
+  std::printf("traj_diff=%.3e rk2_diff=%.3e linf=%.3e sum_drift=%.3e extrap_diff=%.3e "
+              "set_diff=%.3e insitu_extrap_diff=%.3e\n",
+              traj_diff, rk2_diff, err / mx, std::fabs(m1 - m0) / std::fabs(m0), extrap_diff,
+              set_diff, insitu_extrap_diff);
   return 0;
 }

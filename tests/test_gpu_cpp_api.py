@@ -23,3 +23,7 @@ def test_cpp_functor_api_driver():
     assert vals["rk2_diff"] == 0.0
     # rotating blob after 20 steps (1/10 revolution), q = 8, depth 3
     assert vals["linf"] < 2e-3
+    # extrapolating and four-tree functors: fused == host loop
+    assert vals["extrap_diff"] == 0.0
+    assert vals["set_diff"] == 0.0
+    assert vals["insitu_extrap_diff"] < 1e-12

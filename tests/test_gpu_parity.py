@@ -11,7 +11,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import edge_points, random_tree
+from conftest import edge_points, random_tree, wrap_periodic
 
 pytestmark = pytest.mark.gpu
 
@@ -207,8 +207,7 @@ def test_insitu_steps_against_oracle(torch, tb, orc, mode, bc, setup):
         pts = orc.collect_grid_points(oc)
         ref_dep = orc.traj_rk2(ov, pts, step * dt, step * dt - dt, 2, bc, g=fac)
         if bc == 1:
-            ref_dep = np.where(ref_dep < 0, ref_dep + 1, ref_dep)
-            ref_dep = np.where(ref_dep >= 1, ref_dep - 1, ref_dep)
+            ref_dep = wrap_periodic(ref_dep)
         orc.semilag_insitu(ov, oc, M, step, dt, 2, bc, g=fac)
         tb.semilag_insitu(gv, gc, step, dt, 2, bc, mode, g=fac, dep=dep)
         got_dep = dep.cpu().numpy()
@@ -313,8 +312,7 @@ def test_insitu_velocity_functors_against_oracle(torch, tb, orc, kind, mode):
     for step in range(2, 5):
         pts = orc.collect_grid_points(oc)
         ref_dep = orc.traj_rk2_v(ovel, pts, step * dt, step * dt - dt, 2, 1)
-        ref_dep = np.where(ref_dep < 0, ref_dep + 1, ref_dep)
-        ref_dep = np.where(ref_dep >= 1, ref_dep - 1, ref_dep)
+        ref_dep = wrap_periodic(ref_dep)
         orc.semilag_insitu_v(ovel, oc, M, step, dt, 2, 1)
         tb.semilag_insitu(gvel, gc, step, dt, 2, 1, mode, dep=dep)
         if mode == "exact":
@@ -337,3 +335,69 @@ def test_velocity_errors(torch, tb):
         tb.traj_rk2(tb.Velocity.extrap(v3, v3b), x, 0.1, 0.0, 1)
     with pytest.raises(ValueError):
         tb.Velocity.set([v3, v3], [0, 1, 2, 3])
+
+
+@pytest.mark.parametrize("q", [8, 5])
+def test_every_step_shape_matches_oracle(torch, tb, orc, q):
+    # the launch shapes (arrival order, or points regrouped by leaf inside
+    # each workgroup) change which lane evaluates a point, never its value;
+    # q = 5 exercises the fallback of the shapes built for q = 8 and 14 only
+    from tbslas_amd import _lib, fields
+
+    anchor, depth = random_tree(41, max_depth=3)
+    gv, ov = tg_trees(tb, orc, anchor, depth, q)
+    two = lambda p: np.concatenate([fields.gaussian(p, (0.55, 0.5, 0.45), 0.12),
+                                    fields.gaussian(p, (0.4, 0.6, 0.5), 0.2)], axis=1)
+    cf = tb.fit_function(two, anchor, depth, q, 2)
+    oc = orc.Tree(anchor, depth, q, 2, cf)
+    dt = 0.03
+    pts = orc.collect_grid_points(oc)
+    # tinit = timestep * dt, tfinal = tinit - dt (semilag.inc:34-35)
+    ref_dep = orc.traj_rk2(ov, pts, 2 * dt, 2 * dt - dt, 2, 1)
+    ref_dep = wrap_periodic(ref_dep)
+    ref_val, _ = orc.eval_tree(oc, ref_dep, 1)
+    P = (q + 1) ** 3
+    ref_val = ref_val.reshape(len(depth), P, 2).transpose(0, 2, 1)
+    gc = tb.Tree(anchor, depth, q, 2, cf)
+    dep = torch.empty((gc.L * P, 3), dtype=torch.float64, device="cuda")
+    old = _lib.get_option(_lib.OPT_STEP_SHAPE)
+    try:
+        for shape in range(8):
+            _lib.set_option(_lib.OPT_STEP_SHAPE, shape)
+            dep.zero_()
+            vals = tb.semilag_values(gv, gc, 2, dt, 2, 1, "exact", dep=dep)
+            assert np.array_equal(dep.cpu().numpy(), ref_dep), shape
+            assert np.array_equal(vals.cpu().numpy(), ref_val), shape
+            v64 = tb.semilag_values(gv, gc, 2, dt, 2, 1, "fp64").cpu().numpy()
+            assert rel(v64, ref_val) <= FP64_TOL, shape
+    finally:
+        _lib.set_option(_lib.OPT_STEP_SHAPE, old)
+
+
+def test_fp32_head64_mode(torch, tb, orc):
+    # coefficients decaying 10x per total degree, like a well-resolved field:
+    # the FP64 head leaves only the FP32 rounding of the degree > 2 tail,
+    # bounded by a few FP32 ulps of the tail's coefficient sum; the pure
+    # FP32 mode rounds the whole value
+    q = 10
+    anchor, depth = random_tree(51, max_depth=3)
+    deg = np.array([i + j + k for i in range(q + 1) for j in range(q + 1 - i)
+                    for k in range(q + 1 - i - j)])
+    rng = np.random.default_rng(9)
+    coeff = rng.standard_normal((len(depth), 2, tb.ncoef(q))) * 0.1 ** deg
+    g, o = pair(tb, orc, anchor, depth, q, 2, coeff)
+    x = np.random.default_rng(8).random((100000, 3))
+    ref, _ = orc.eval_tree(o, x, 0)
+    scale = np.abs(ref).max()
+    e32 = np.abs(g.eval(dev(torch, x), 0, "fp32").cpu().numpy() - ref).max() / scale
+    e32h = np.abs(g.eval(dev(torch, x), 0, "fp32h").cpu().numpy() - ref).max() / scale
+    tail = np.abs(coeff[..., deg > 2]).sum(-1).max() / scale
+    assert e32 <= FP32_TOL
+    assert e32h <= 8 * 2.0 ** -24 * tail + 1e-14, (e32h, tail)
+    assert e32h * 100 <= e32, (e32, e32h)
+    # slowly decaying coefficients still meet the FP32 bar
+    coeff = rng.standard_normal(coeff.shape) / (1 + np.arange(coeff.shape[-1]))
+    g.set_coeffs(coeff)
+    o2 = orc.Tree(anchor, depth, q, 2, coeff)
+    ref, _ = orc.eval_tree(o2, x, 0)
+    assert rel(g.eval(dev(torch, x), 0, "fp32h").cpu().numpy(), ref) <= FP32_TOL
