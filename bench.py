@@ -149,6 +149,10 @@ def main():
     ap.add_argument("--cpu-leaves", type=int, default=int(os.environ.get("TBX_CPU_LEAVES", 0)),
                     help="leaves in the CPU baseline step (0: all)")
     ap.add_argument("--shape", type=int, default=None, help="step kernel launch shape")
+    ap.add_argument("--profile", default=None, metavar="MODE",
+                    help="counter-collection run: a 1 GiB device copy (the reference "
+                         "for byte counters), then --steps step kernels of one "
+                         "arithmetic mode (fp64, fp32, fp32h, exact); no JSON line")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -174,6 +178,18 @@ def main():
     if args.shape is not None:
         _lib.set_option(_lib.OPT_STEP_SHAPE, args.shape)
     shape = _lib.get_option(_lib.OPT_STEP_SHAPE)
+
+    if args.profile:
+        src = torch.full((1 << 27,), 1.0, dtype=torch.float64, device="cuda")
+        torch.empty_like(src).copy_(src)
+        conc = tb.Tree(anchor, depth, Q, 1, ccf)
+        vals = torch.empty((conc.L, 1, conc.P), dtype=torch.float64, device="cuda")
+        for s in range(1, args.steps + 1):
+            tb.semilag_values(vel, conc, s, DT, 1, tb.BC_PERIODIC, args.profile, g=tfac,
+                              vals=vals)
+            torch.cuda.synchronize()
+            print(f"# {args.profile} step {s} done", file=sys.stderr, flush=True)
+        return
 
     # L-inf of each FP32 mode against FP64 after one step, at every grid point
     first = {}

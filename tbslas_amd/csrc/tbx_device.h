@@ -167,10 +167,15 @@ struct Chunk<float> {
   static constexpr int N = 16;
 };
 
+// volatile: a plain constant-address-space load is "invariant" and the
+// register allocator may rematerialise it at its use, which puts every
+// block's load right before its FMAs and waits a full scalar-load round
+// trip per block; a volatile load stays where the schedule put it, one
+// block ahead.
 template <typename T>
 __device__ __forceinline__ typename Chunk<T>::V load_chunk(const T *p) {
   typedef typename Chunk<T>::V V;
-  return *(const __attribute__((address_space(4))) V *)(p);
+  return *(const volatile __attribute__((address_space(4))) V *)(p);
 }
 
 template <typename V, typename P>
@@ -183,9 +188,13 @@ __device__ __forceinline__ void fence_block(V &a, V &b, P &ptr, T &w, T &v, T &u
   asm volatile("" : "+s"(a), "+s"(b), "+s"(ptr), "+v"(w), "+v"(v), "+v"(u));
 }
 
+// T_i(z) for the outer index is generated inside the contraction by the
+// recurrence of cheb_poly, with the same operations, from zx (the clamped
+// coordinate, 0 outside [-1,1]) and z0 (T_0: 1, or 0 outside): the values
+// are those cheb_poly would store, and the 3 x D basis array shrinks to 2 x D.
 template <int D, int MODE, typename T>
 __device__ __forceinline__ T contract(const T *c, const T *px, const T *py,
-                                      const T *pz) {
+                                      double zx, double z0) {
   typedef typename Chunk<T>::V V;
   constexpr int CN = Chunk<T>::N;  // coefficients per chunk
   constexpr int BN = 2 * CN;       // coefficients per block
@@ -202,8 +211,20 @@ __device__ __forceinline__ T contract(const T *c, const T *px, const T *py,
   }
   T u = T(0);
   int idx = 0;
+  const double z2 = 2.0 * zx;
+  double zm = z0, zc = zx;  // T_{i-1}, T_i for i >= 1
 #pragma unroll
   for (int i = 0; i < D; ++i) {
+    double zi;
+    if (i == 0) {
+      zi = z0;
+    } else if (i == 1) {
+      zi = zx;
+    } else {
+      zi = z2 * zc - zm;  // (2.0 * xx) * y1 - y0, as cheb_poly
+      zm = zc;
+      zc = zi;
+    }
     T v = T(0);
 #pragma unroll
     for (int j = 0; i + j < D; ++j) {
@@ -230,7 +251,7 @@ __device__ __forceinline__ T contract(const T *c, const T *px, const T *py,
       }
       v = madd<MODE>(py[j], w, v);
     }
-    u = madd<MODE>(pz[i], v, u);
+    u = madd<MODE>((T)zi, v, u);
   }
   return u;
 }
@@ -280,7 +301,8 @@ template <int D, int MODE>
 struct PointBasis {
   using T = typename ModeType<MODE>::T;
   int leaf;
-  T px[D], py[D], pz[D];
+  T px[D], py[D];
+  double zx, z0;               // T_i(z) source for contract
   double hx[3], hy[3], hz[3];  // kFp32h: T_0..T_2 in FP64
 
   // hint: the point's leaf when the caller has located it already, else -1
@@ -291,7 +313,9 @@ struct PointBasis {
     leaf_xi(t, leaf, x, y, z, ex, ey, ez);
     cheb_poly<D>(ex, px);
     cheb_poly<D>(ey, py);
-    cheb_poly<D>(ez, pz);
+    const bool zin = fabs(ez) <= 1.0;  // cheb_poly's range test
+    zx = zin ? ez : 0.0;
+    z0 = zin ? 1.0 : 0.0;
     if constexpr (MODE == kFp32h) {
       cheb_poly<3>(ex, hx);
       cheb_poly<3>(ey, hy);
@@ -344,7 +368,7 @@ __device__ __forceinline__ double leaf_value(const TreeView &t, int lf, int comp
                                              const PointBasis<D, MODE> &b) {
   constexpr int NC = D * (D + 1) * (D + 2) / 6;
   const size_t off = ((size_t)lf * t.dof + comp) * NC;
-  const double u = (double)contract<D, MODE>(coeff_base<MODE>(t) + off, b.px, b.py, b.pz);
+  const double u = (double)contract<D, MODE>(coeff_base<MODE>(t) + off, b.px, b.py, b.zx, b.z0);
   if constexpr (MODE == kFp32h)
     return u + head_contract<D>(t.coeff + off, b.hx, b.hy, b.hz);
   else
@@ -409,56 +433,59 @@ __device__ __forceinline__ double cubic1d(double x, const double *xx, double p0,
 // after the first (the caller wrapped it for the first), and the caller's
 // point ends up wrapped once per tree.  This matters only for points more
 // than one period outside the domain.
-template <int D, int MODE>
+// KIND is the launch's vs.kind as a compile-time constant: the composite
+// kinds keep up to four trees' values live, which a single-tree kernel must
+// not pay for in registers.
+template <int D, int MODE, int KIND>
 __device__ __forceinline__ void eval_velocity(const VelSource &vs, int stage, double ts,
                                               int scaled, double g, int bc, double &x,
                                               double &y, double &z, double &v0, double &v1,
                                               double &v2, int hint0 = -1) {
-  if (vs.kind == kVelTree) {
+  if constexpr (KIND == kVelTree) {
     eval_vec3<D, MODE>(vs.t[0], x, y, z, v0, v1, v2, hint0);
     if (scaled) {
       v0 = v0 * g;
       v1 = v1 * g;
       v2 = v2 * g;
     }
-    return;
-  }
-  const int nt = vs.kind == kVelExtrap ? (stage == 1 ? 1 : 2) : 4;
-  double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
-  double c0 = 0, c1 = 0, c2 = 0, d0 = 0, d1 = 0, d2 = 0;
-#pragma unroll 1
-  for (int k = 0; k < nt; ++k) {
-    if (k > 0 && bc == TBX_BC_PERIODIC) {
-      x = wrap1(x);
-      y = wrap1(y);
-      z = wrap1(z);
-    }
-    const TreeView &tk = k == 0 ? vs.t[0] : k == 1 ? vs.t[1] : k == 2 ? vs.t[2] : vs.t[3];
-    double r0, r1, r2;
-    eval_vec3<D, MODE>(tk, x, y, z, r0, r1, r2, k == 0 ? hint0 : -1);
-    if (k == 0) {
-      a0 = r0; a1 = r1; a2 = r2;
-    } else if (k == 1) {
-      b0 = r0; b1 = r1; b2 = r2;
-    } else if (k == 2) {
-      c0 = r0; c1 = r1; c2 = r2;
-    } else {
-      d0 = r0; d1 = r1; d2 = r2;
-    }
-  }
-  if (vs.kind == kVelExtrap) {
-    if (stage == 1) {
-      v0 = a0; v1 = a1; v2 = a2;
-    } else {
-      const double ccoeff = 3.0 / 2, pcoeff = 0.5;
-      v0 = ccoeff * a0 - pcoeff * b0;
-      v1 = ccoeff * a1 - pcoeff * b1;
-      v2 = ccoeff * a2 - pcoeff * b2;
-    }
   } else {
-    v0 = cubic1d(ts, vs.tt, a0, b0, c0, d0);
-    v1 = cubic1d(ts, vs.tt, a1, b1, c1, d1);
-    v2 = cubic1d(ts, vs.tt, a2, b2, c2, d2);
+    const int nt = KIND == kVelExtrap ? (stage == 1 ? 1 : 2) : 4;
+    double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+    double c0 = 0, c1 = 0, c2 = 0, d0 = 0, d1 = 0, d2 = 0;
+#pragma unroll 1
+    for (int k = 0; k < nt; ++k) {
+      if (k > 0 && bc == TBX_BC_PERIODIC) {
+        x = wrap1(x);
+        y = wrap1(y);
+        z = wrap1(z);
+      }
+      const TreeView &tk = k == 0 ? vs.t[0] : k == 1 ? vs.t[1] : k == 2 ? vs.t[2] : vs.t[3];
+      double r0, r1, r2;
+      eval_vec3<D, MODE>(tk, x, y, z, r0, r1, r2, k == 0 ? hint0 : -1);
+      if (k == 0) {
+        a0 = r0; a1 = r1; a2 = r2;
+      } else if (k == 1) {
+        b0 = r0; b1 = r1; b2 = r2;
+      } else if (k == 2) {
+        c0 = r0; c1 = r1; c2 = r2;
+      } else {
+        d0 = r0; d1 = r1; d2 = r2;
+      }
+    }
+    if constexpr (KIND == kVelExtrap) {
+      if (stage == 1) {
+        v0 = a0; v1 = a1; v2 = a2;
+      } else {
+        const double ccoeff = 3.0 / 2, pcoeff = 0.5;
+        v0 = ccoeff * a0 - pcoeff * b0;
+        v1 = ccoeff * a1 - pcoeff * b1;
+        v2 = ccoeff * a2 - pcoeff * b2;
+      }
+    } else {
+      v0 = cubic1d(ts, vs.tt, a0, b0, c0, d0);
+      v1 = cubic1d(ts, vs.tt, a1, b1, c1, d1);
+      v2 = cubic1d(ts, vs.tt, a2, b2, c2, d2);
+    }
   }
 }
 
@@ -466,7 +493,7 @@ __device__ __forceinline__ void eval_velocity(const VelSource &vs, int stage, do
 // first evaluation wraps x itself (EvalTree mutates the caller's array,
 // and ComputeTrajRK2 passes xsol as xinit); the midpoint is wrapped only
 // for its evaluation.
-template <int D, int MODE>
+template <int D, int MODE, int KIND>
 __device__ __forceinline__ void rk2_substep(const VelSource &vel, int bc,
                                             double htau, double tau, int scaled,
                                             double g1, double g2, double ts1,
@@ -478,7 +505,7 @@ __device__ __forceinline__ void rk2_substep(const VelSource &vel, int bc,
     z = wrap1(z);
   }
   double v0, v1, v2;
-  eval_velocity<D, MODE>(vel, 1, ts1, scaled, g1, bc, x, y, z, v0, v1, v2);
+  eval_velocity<D, MODE, KIND>(vel, 1, ts1, scaled, g1, bc, x, y, z, v0, v1, v2);
   // xtmp = xinit + (0.5 * dt) * vtmp   (traj.inc:36)
   double hx = x + htau * v0, hy = y + htau * v1, hz = z + htau * v2;
   if (bc == TBX_BC_PERIODIC) {
@@ -486,7 +513,7 @@ __device__ __forceinline__ void rk2_substep(const VelSource &vel, int bc,
     hy = wrap1(hy);
     hz = wrap1(hz);
   }
-  eval_velocity<D, MODE>(vel, 2, ts2, scaled, g2, bc, hx, hy, hz, v0, v1, v2);
+  eval_velocity<D, MODE, KIND>(vel, 2, ts2, scaled, g2, bc, hx, hy, hz, v0, v1, v2);
   // xsol = xinit + dt * vtmp   (traj.inc:41-42)
   x = x + tau * v0;
   y = y + tau * v1;

@@ -747,7 +747,7 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
   const int P = conc->P();
   StepArgs a{};
   const int want = g_step_shape.load();
-  a.shape = step_shape_built(want, conc->q) ? want : kDefaultStepShape;
+  a.shape = vs.kind == kVelTree && step_shape_built(want, conc->q) ? want : kDefaultStepShape;
   a.npts = conc->L * P;
   a.nblocks = (a.npts + kStepShapeBlock[a.shape] - 1) / kStepShapeBlock[a.shape];
   a.per_xcd = (a.nblocks + 7) / 8;

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(TreeView t,
 // A velocity functor (FieldExtrapFunctor / FieldSetFunctor / a scaled tree)
 // at caller points at time t; the extrapolating kind evaluates its
 // extrapolation (the second-stage functor).
-template <int MODE>
+template <int MODE, int KIND>
 __global__ __launch_bounds__(kBlock) void k_veval(VelSource vel,
                                                   double *__restrict__ pts,
                                                   int64_t n, int bc,
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_veval(VelSource vel,
     z = wrap1(z);
   }
   double v0, v1, v2;
-  eval_velocity<D, MODE>(vel, 2, t, scaled, g, bc, x, y, z, v0, v1, v2);
+  eval_velocity<D, MODE, KIND>(vel, 2, t, scaled, g, bc, x, y, z, v0, v1, v2);
   if (bc == TBX_BC_PERIODIC && write_back) {  // wrapped once per tree
     pts[3 * i + 0] = x;
     pts[3 * i + 1] = y;
@@ -73,14 +73,14 @@ __global__ __launch_bounds__(kBlock) void k_veval(VelSource vel,
 }
 
 // One IntegrateRK2 substep for caller points, in place.
-template <int MODE>
+template <int MODE, int KIND>
 __global__ __launch_bounds__(kBlock) void k_traj(VelSource vel,
                                                  double *__restrict__ xs,
                                                  int64_t n, SubArgs a) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   double x = xs[3 * i + 0], y = xs[3 * i + 1], z = xs[3 * i + 2];
-  rk2_substep<D, MODE>(vel, a.bc, a.htau, a.tau, a.scaled, a.g1, a.g2, a.ts1,
+  rk2_substep<D, MODE, KIND>(vel, a.bc, a.htau, a.tau, a.scaled, a.g1, a.g2, a.ts1,
                        a.ts2, x, y, z);
   xs[3 * i + 0] = x;
   xs[3 * i + 1] = y;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_traj(VelSource vel,
 // minimum waves per SIMD the register budget must allow) select the launch
 // shape: wide workgroups keep the waves resident on a CU inside one or two
 // leaves, so their scalar coefficient loads hit the same cache lines.
-template <int MODE, int BLOCK, int WPE>
+template <int MODE, int KIND, int BLOCK, int WPE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_step(
     VelSource vel, TreeView conc, StepArgs a, double *__restrict__ vals,
     double *__restrict__ dep) {
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
   double z = (double)inf.z * len + len * a.node[m / (D * D)];
   // ComputeTrajRK2 (traj.inc:48-68), tinit -> tfinal = tinit - dt
   for (int s = 0; s < a.nsub; ++s)
-    rk2_substep<D, MODE>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[2 * s],
+    rk2_substep<D, MODE, KIND>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[2 * s],
                          a.g[2 * s + 1], a.ts[2 * s], a.ts[2 * s + 1], x, y, z);
   // con_evaluator(departure points) wraps them in place (semilag.inc:43)
   if (a.bc == TBX_BC_PERIODIC) {
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
 // (start point, wrapped midpoint) travels with the point through LDS, and
 // the leaf found for the regroup is the one the evaluation uses.  Lanes
 // past the last arrival point carry a copy of it and store nothing.
-template <int MODE, int BLOCK, int WPE>
+template <int MODE, int KIND, int BLOCK, int WPE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_step_rg(
     VelSource vel, TreeView conc, StepArgs a, double *__restrict__ vals,
     double *__restrict__ dep) {
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
         st[2] = p3[2];
       }
       double v0, v1, v2;
-      eval_velocity<D, MODE>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, st[0], st[1],
+      eval_velocity<D, MODE, KIND>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, st[0], st[1],
                              st[2], v0, v1, v2, lf);
       // midpoint xtmp = xinit + (0.5 * dt) * vtmp (traj.inc:36), wrapped
       // for its evaluation only
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
       }
       lf = locate(vel.t[0], st[3], st[4], st[5]);
       regroup(sh, orig, lf, st);
-      eval_velocity<D, MODE>(vel, 2, a.ts[2 * s + 1], a.scaled, a.g[2 * s + 1], a.bc, st[3],
+      eval_velocity<D, MODE, KIND>(vel, 2, a.ts[2 * s + 1], a.scaled, a.g[2 * s + 1], a.bc, st[3],
                              st[4], st[5], v0, v1, v2, lf);
       // xsol = xinit + dt * vtmp (traj.inc:41-42)
       st[0] = st[0] + a.tau * v0;
@@ -246,46 +246,68 @@ hipError_t with_mode(int mode, F f) {
   }
 }
 
-template <int MODE, int BLOCK, int WPE>
+// Calls f with the velocity kind as a compile-time constant.
+template <typename F>
+hipError_t with_kind(int kind, F f) {
+  switch (kind) {
+    case kVelTree:
+      return f(std::integral_constant<int, kVelTree>{});
+    case kVelExtrap:
+      return f(std::integral_constant<int, kVelExtrap>{});
+    case kVelSet:
+      return f(std::integral_constant<int, kVelSet>{});
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <int MODE, int KIND, int BLOCK, int WPE>
 hipError_t launch_step_shape(const VelSource &vel, const TreeView &conc, const StepArgs &a,
                              double *vals, double *dep, hipStream_t s) {
-  hipLaunchKernelGGL((k_step<MODE, BLOCK, WPE>), dim3((unsigned)(8 * a.per_xcd)), dim3(BLOCK), 0,
-                     s, vel, conc, a, vals, dep);
+  hipLaunchKernelGGL((k_step<MODE, KIND, BLOCK, WPE>), dim3((unsigned)(8 * a.per_xcd)),
+                     dim3(BLOCK), 0, s, vel, conc, a, vals, dep);
   return hipGetLastError();
 }
 
-template <int MODE, int BLOCK, int WPE>
+template <int MODE, int KIND, int BLOCK, int WPE>
 hipError_t launch_step_rg_shape(const VelSource &vel, const TreeView &conc, const StepArgs &a,
                                 double *vals, double *dep, hipStream_t s) {
-  hipLaunchKernelGGL((k_step_rg<MODE, BLOCK, WPE>), dim3((unsigned)(8 * a.per_xcd)), dim3(BLOCK),
-                     0, s, vel, conc, a, vals, dep);
+  hipLaunchKernelGGL((k_step_rg<MODE, KIND, BLOCK, WPE>), dim3((unsigned)(8 * a.per_xcd)),
+                     dim3(BLOCK), 0, s, vel, conc, a, vals, dep);
   return hipGetLastError();
 }
 
 template <int MODE>
 hipError_t launch_step_mode(const VelSource &vel, const TreeView &conc, const StepArgs &a,
                             double *vals, double *dep, hipStream_t s) {
+  // the composite velocity kinds run the default shape only (the host
+  // sizes the grid for the shape it asks for: kDefaultStepShape then)
+  if (vel.kind == kVelExtrap)
+    return launch_step_rg_shape<MODE, kVelExtrap, 1024, 4>(vel, conc, a, vals, dep, s);
+  if (vel.kind == kVelSet)
+    return launch_step_rg_shape<MODE, kVelSet, 1024, 4>(vel, conc, a, vals, dep, s);
   // launch shapes of kStepShapes (tbx_internal.h); the extra shapes are
   // built for the degrees of the benchmark workloads only
+  constexpr int T = kVelTree;
   switch (a.shape) {
     case 0:
-      return launch_step_shape<MODE, 256, 1>(vel, conc, a, vals, dep, s);
+      return launch_step_shape<MODE, T, 256, 1>(vel, conc, a, vals, dep, s);
 #if TBX_Q == 8 || TBX_Q == 14
     case 4:
-      return launch_step_rg_shape<MODE, 256, 2>(vel, conc, a, vals, dep, s);
+      return launch_step_rg_shape<MODE, T, 256, 2>(vel, conc, a, vals, dep, s);
     case 1:
-      return launch_step_shape<MODE, 256, 4>(vel, conc, a, vals, dep, s);
+      return launch_step_shape<MODE, T, 256, 4>(vel, conc, a, vals, dep, s);
     case 2:
-      return launch_step_shape<MODE, 1024, 4>(vel, conc, a, vals, dep, s);
+      return launch_step_shape<MODE, T, 1024, 4>(vel, conc, a, vals, dep, s);
     case 3:
-      return launch_step_shape<MODE, 512, 4>(vel, conc, a, vals, dep, s);
+      return launch_step_shape<MODE, T, 512, 4>(vel, conc, a, vals, dep, s);
     case 5:
-      return launch_step_rg_shape<MODE, 512, 2>(vel, conc, a, vals, dep, s);
+      return launch_step_rg_shape<MODE, T, 512, 2>(vel, conc, a, vals, dep, s);
     case 7:
-      return launch_step_rg_shape<MODE, 512, 4>(vel, conc, a, vals, dep, s);
+      return launch_step_rg_shape<MODE, T, 512, 4>(vel, conc, a, vals, dep, s);
 #endif
     default:  // shape 6, and the shapes not built for this degree
-      return launch_step_rg_shape<MODE, 1024, 4>(vel, conc, a, vals, dep, s);
+      return launch_step_rg_shape<MODE, T, 1024, 4>(vel, conc, a, vals, dep, s);
   }
 }
 
@@ -313,9 +335,11 @@ hipError_t TBX_CAT(launch_veval_q, TBX_Q)(int mode, const VelSource &vel,
                                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
   return with_mode(mode, [&](auto m) {
-    hipLaunchKernelGGL(k_veval<decltype(m)::value>, dim3(grid_for(n)), dim3(kBlock), 0, s, vel,
-                       pts, n, bc, write_back, t, scaled, g, out);
-    return hipGetLastError();
+    return with_kind(vel.kind, [&](auto k) {
+      hipLaunchKernelGGL((k_veval<decltype(m)::value, decltype(k)::value>), dim3(grid_for(n)),
+                         dim3(kBlock), 0, s, vel, pts, n, bc, write_back, t, scaled, g, out);
+      return hipGetLastError();
+    });
   });
 }
 
@@ -324,9 +348,11 @@ hipError_t TBX_CAT(launch_traj_q, TBX_Q)(int mode, const VelSource &vel,
                                          const SubArgs &a, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   return with_mode(mode, [&](auto m) {
-    hipLaunchKernelGGL(k_traj<decltype(m)::value>, dim3(grid_for(n)), dim3(kBlock), 0, s, vel,
-                       x, n, a);
-    return hipGetLastError();
+    return with_kind(vel.kind, [&](auto k) {
+      hipLaunchKernelGGL((k_traj<decltype(m)::value, decltype(k)::value>), dim3(grid_for(n)),
+                         dim3(kBlock), 0, s, vel, x, n, a);
+      return hipGetLastError();
+    });
   });
 }
 
