@@ -131,14 +131,16 @@ def cpu_baseline(anchor, depth, ccf, vcf, n_leaves: int):
 
 def traffic_from_profiles(kernel: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
-    summary, if one exists."""
-    p = os.path.join(ROOT, "profiles", "step_traffic.json")
-    if not os.path.exists(p):
-        return None
+    summary (profiles/step_traffic.txt: "kernel bytes" lines), if measured."""
+    p = os.path.join(ROOT, "profiles", "step_traffic.txt")
     try:
-        return json.load(open(p)).get(kernel)
-    except (OSError, ValueError):
-        return None
+        for line in open(p):
+            f = line.split()
+            if len(f) == 2 and not line.startswith("#") and f[0] == kernel:
+                return float(f[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():

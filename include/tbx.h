@@ -202,6 +202,39 @@ int tbx_semilag_insitu_v(const tbx_velocity *vel, tbx_tree *conc, int timestep,
                          double dt, int num_rk_step, int bc, uint32_t flags,
                          double *dep, tbx_stream s);
 
+/* ---- adjoint (no counterpart in the reference) --------------------------
+ * The step c' = R E_D(V) c of tbx_semilag_insitu (one RK2 substep, a single
+ * velocity tree, optionally scaled by g) is linear in c; its cotangents are
+ *   dc = E_D^T R^T lam'                the transposed interpolation, a
+ *                                      per-leaf scatter-add into coefficients
+ *   dV = sum_p a_p . dD_p/dV           basis derivatives at the departure
+ *                                      points chained through the RK2 stage
+ * Arithmetic: TBX_FP64 or TBX_EXACT (the forward's trajectory arithmetic);
+ * the sums are atomic, so results are reproducible up to summation order. */
+
+/* Cotangent of one tbx_semilag_insitu step taken from conc's current
+ * coefficients c (left unchanged): lam_out = dL/dc given lam_next = dL/dc'
+ * (device arrays [L][dof][Nc]); grad_vel (device [L_vel][3][Nc_vel], may be
+ * NULL) is ADDED to.  num_rk_step must be 1. */
+int tbx_semilag_insitu_vjp(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
+                           const tbx_tree *conc, int timestep, double dt,
+                           int num_rk_step, int bc, uint32_t flags,
+                           const double *lam_next, double *lam_out,
+                           double *grad_vel, tbx_stream s);
+
+/* nsteps in-situ steps from conc's coefficients c_0 (timesteps 1..nsteps),
+ * loss = 0.5 |c_T - c_target|^2 (returned on the host), and its gradients:
+ * grad_c0 [L][dof][Nc] and grad_vel [L_vel][3][Nc_vel] (may be NULL),
+ * overwritten.  States are checkpointed every checkpoint_every steps and
+ * recomputed segment by segment in the backward sweep; a stationary
+ * velocity (g NULL) scatters its cotangent once.  conc holds c_T on
+ * return. */
+int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
+                    tbx_tree *conc, int nsteps, double dt, int bc,
+                    uint32_t flags, const double *c_target, int checkpoint_every,
+                    double *grad_c0, double *grad_vel, double *loss,
+                    tbx_stream s);
+
 /* ---- host helpers (setup; no device work) ----------------------------- */
 /* new_nodes (cheb.h:41-58), x[q+1]. */
 int tbx_new_nodes(int q, double *x);

@@ -88,6 +88,10 @@ _SIGS = {
                                     _P, _P]),
     "tbx_semilag_insitu_v": (_I32, [C.POINTER(_Velocity), _P, _I32, _D, _I32, _I32, _U32, _P,
                                     _P]),
+    "tbx_semilag_insitu_vjp": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P,
+                                      _P, _P]),
+    "tbx_adjoint_run": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _U32, _P, _I32, _P, _P,
+                               C.POINTER(C.c_double), _P]),
     "tbx_new_nodes": (_I32, [_I32, _P]),
     "tbx_pt2coeff_matrix": (_I32, [_I32, _P]),
     "tbx_uniform_leaves": (_I32, [_I32, _P, _P]),

@@ -56,6 +56,8 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     for q in range(1, MAX_Q + 1):
         units.append((os.path.join(CSRC, "tbx_kernels.hip"),
                       os.path.join(BUILD, f"tbx_kernels_q{q}.o"), [f"-DTBX_Q={q}"]))
+        units.append((os.path.join(CSRC, "tbx_adjoint.hip"),
+                      os.path.join(BUILD, f"tbx_adjoint_q{q}.o"), [f"-DTBX_Q={q}"]))
     units.append((os.path.join(CSRC, "tbx_common.hip"), os.path.join(BUILD, "tbx_common.o"), []))
     units.append((os.path.join(CSRC, "tbx_host.cpp"), os.path.join(BUILD, "tbx_host.o"),
                   ["-x", "hip"]))

@@ -1,0 +1,346 @@
+// Adjoint of one SolveSemilagInSitu step for one Chebyshev degree
+// (compiled once per degree with -DTBX_Q=<q>, like tbx_kernels.hip).
+//
+// The forward step is c' = R E_D(V) c: E_D interpolates c at the departure
+// points D(V) of the grid points (RK2 through the velocity tree V), R is the
+// per-leaf refit.  For a cotangent lam' of c':
+//   dc = E_D^T R^T lam'          transposed interpolation: every departure
+//                                point adds u_p B_m(D_p) into the
+//                                coefficients of its leaf (u = R^T lam')
+//   dV = sum_p a_p . dD_p/dV     a_p = sum_c u_{p,c} grad c_c(D_p), chained
+//                                through the midpoint RK2 substep
+// R^T lam' is a GEMM on the host side.  The reference has no adjoint; these
+// kernels differentiate exactly what the forward kernels compute (the same
+// trajectory arithmetic, the same leaf assignment), for one RK2 substep and
+// a single velocity tree.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "tbx_device.h"
+
+#ifndef TBX_Q
+#error "compile with -DTBX_Q=<degree>"
+#endif
+
+namespace tbx {
+namespace {
+
+constexpr int D = TBX_Q + 1;
+constexpr int NC = D * (D + 1) * (D + 2) / 6;
+
+// T_0..T_{D-1} and dT/dxi by differentiating the three-term recurrence,
+// dT_{k+1} = 2 T_k + 2 xi dT_k - dT_{k-1}; zeros outside [-1,1] like
+// cheb_poly.
+__device__ __forceinline__ void cheb_poly_d(double x, double *p, double *dp) {
+  const bool in = fabs(x) <= 1.0;
+  const double xx = in ? x : 0.0;
+  p[0] = in ? 1.0 : 0.0;
+  dp[0] = 0.0;
+  if constexpr (D > 1) {
+    p[1] = xx;
+    dp[1] = p[0];
+  }
+#pragma unroll
+  for (int k = 2; k < D; ++k) {
+    p[k] = 2.0 * xx * p[k - 1] - p[k - 2];
+    dp[k] = 2.0 * p[k - 1] + 2.0 * xx * dp[k - 1] - dp[k - 2];
+  }
+}
+
+// Basis and its derivatives at one point, in the point's leaf.
+struct AdjBasis {
+  int leaf;
+  double s2;  // d xi / dx = 2 * 2^depth
+  double tx[D], ty[D], tz[D], dx[D], dy[D], dz[D];
+
+  __device__ __forceinline__ void build(const TreeView &t, double x, double y, double z,
+                                        int hint = -1) {
+    leaf = hint >= 0 ? hint : locate(t, x, y, z);
+    double ex, ey, ez;
+    leaf_xi(t, leaf, x, y, z, ex, ey, ez);
+    s2 = 2.0 * (double)(1u << t.info[leaf].w);
+    cheb_poly_d(ex, tx, dx);
+    cheb_poly_d(ey, ty, dy);
+    cheb_poly_d(ez, tz, dz);
+  }
+};
+
+// sum_m c_m bx_k by_j bz_i over the packed triangle, c wave-uniform.
+__device__ __forceinline__ double contract3(const double *c, const double *bx,
+                                            const double *by, const double *bz) {
+  double u = 0.0;
+  int idx = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; i + j < D; ++j) {
+      double w = 0.0;
+#pragma unroll
+      for (int k = 0; i + j + k < D; ++k) w = __builtin_fma(bx[k], load_uniform(c + idx++), w);
+      v = __builtin_fma(by[j], w, v);
+    }
+    u = __builtin_fma(bz[i], v, u);
+  }
+  return u;
+}
+
+// Gradient (d/dx, d/dy, d/dz) of one component at the point of b, c the
+// component's coefficients in b's leaf.
+__device__ __forceinline__ void grad3(const double *c, const AdjBasis &b, double &gx,
+                                      double &gy, double &gz) {
+  gx = contract3(c, b.dx, b.ty, b.tz) * b.s2;
+  gy = contract3(c, b.tx, b.dy, b.tz) * b.s2;
+  gz = contract3(c, b.tx, b.ty, b.dz) * b.s2;
+}
+
+// Sum of v over the 64 lanes, in lane 63 (all lanes must be active):
+// row prefix sums by DPP row shifts, then the row totals combined by the
+// row broadcasts.  Lanes whose source is outside the row read 0.
+__device__ __forceinline__ double dpp_shift(double v, int ctrl_id) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  int rlo, rhi;
+  switch (ctrl_id) {
+    case 0:
+      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x111, 0xf, 0xf, true);
+      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x111, 0xf, 0xf, true);
+      break;
+    case 1:
+      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x112, 0xf, 0xf, true);
+      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x112, 0xf, 0xf, true);
+      break;
+    case 2:
+      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x114, 0xf, 0xf, true);
+      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x114, 0xf, 0xf, true);
+      break;
+    case 3:
+      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x118, 0xf, 0xf, true);
+      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x118, 0xf, 0xf, true);
+      break;
+    case 4:  // row_bcast:15 into rows 1 and 3
+      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x142, 0xa, 0xf, false);
+      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x142, 0xa, 0xf, false);
+      break;
+    default:  // row_bcast:31 into rows 2 and 3
+      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x143, 0xc, 0xf, false);
+      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x143, 0xc, 0xf, false);
+      break;
+  }
+  return __hiloint2double(rhi, rlo);
+}
+
+__device__ __forceinline__ double wave_sum63(double v) {
+  v = v + dpp_shift(v, 0);
+  v = v + dpp_shift(v, 1);
+  v = v + dpp_shift(v, 2);
+  v = v + dpp_shift(v, 3);
+  v = v + dpp_shift(v, 4);
+  v = v + dpp_shift(v, 5);
+  return v;
+}
+
+// Transposed interpolation of ncomp weighted components: for every leaf
+// held by a lane of the wave, G[leaf][c][m] += sum over its lanes of
+// w(c) B_m(point).  All lanes of the wave must be active; each lane's basis
+// b is in its own leaf.  One atomic add per coefficient and leaf per wave.
+template <typename W>
+__device__ __forceinline__ void scatter_transposed(double *G, int ncomp, const AdjBasis &b,
+                                                   W weight) {
+  const bool last = (threadIdx.x & 63) == 63;
+  uint64_t pending = __builtin_amdgcn_ballot_w64(true);
+  while (pending) {
+    const int lf = __builtin_amdgcn_readlane(b.leaf, __builtin_ctzll(pending));
+    const bool mine = b.leaf == lf;
+    pending &= ~__builtin_amdgcn_ballot_w64(mine);
+#pragma unroll 1
+    for (int c = 0; c < ncomp; ++c) {
+      const double wc = mine ? weight(c) : 0.0;
+      double *g = G + ((size_t)lf * ncomp + c) * NC;
+      int idx = 0;
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+#pragma unroll
+        for (int j = 0; i + j < D; ++j) {
+          const double tij = wc * b.tz[i] * b.ty[j];
+#pragma unroll
+          for (int k = 0; i + j + k < D; ++k) {
+            const double s = wave_sum63(tij * b.tx[k]);
+            if (last) atomicAdd(g + idx, s);
+            ++idx;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Arrival point of global grid index gi (tree_utils.h:480-491).
+__device__ __forceinline__ void arrival_point(const TreeView &conc, const StepArgs &a,
+                                              int64_t gi, double &x, double &y, double &z) {
+  const int64_t leaf = gi / a.P;
+  const int m = (int)(gi - leaf * a.P);
+  const uint4 inf = conc.info[leaf];
+  const double len = 1.0 / (double)(1u << inf.w);
+  x = (double)inf.x * len + len * a.node[m % D];
+  y = (double)inf.y * len + len * a.node[(m / D) % D];
+  z = (double)inf.z * len + len * a.node[m / (D * D)];
+}
+
+// Cotangent of c and, when A is given, the per-point velocity cotangent
+// factor a_p += sum_c u_{p,c} grad c_c(D_p).  ua[al][c][am] = (R^T lam')
+// at arrival point (al, am).  MODE is the forward's trajectory arithmetic.
+template <int MODE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_adj_conc(VelSource vel, TreeView conc, StepArgs a,
+                                                    const double *__restrict__ ua,
+                                                    double *__restrict__ lam,
+                                                    double *__restrict__ A) {
+  __shared__ RegroupLds<BLOCK, 3> sh;
+  const uint32_t b = blockIdx.x;
+  const int64_t lb = (int64_t)(b & 7u) * a.per_xcd + (int64_t)(b >> 3);
+  if (lb >= a.nblocks) return;  // uniform over the workgroup
+  const int64_t i0 = lb * BLOCK;
+  int orig = (int)threadIdx.x;
+  double p3[3];
+  {
+    const int64_t i = i0 + orig < a.npts ? i0 + orig : a.npts - 1;
+    double x, y, z;
+    arrival_point(conc, a, i, x, y, z);
+    rk2_substep<D, MODE, kVelTree>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[0], a.g[1], a.ts[0],
+                                   a.ts[1], x, y, z);
+    if (a.bc == TBX_BC_PERIODIC) {
+      x = wrap1(x);
+      y = wrap1(y);
+      z = wrap1(z);
+    }
+    p3[0] = x;
+    p3[1] = y;
+    p3[2] = z;
+  }
+  int lf = locate(conc, p3[0], p3[1], p3[2]);
+  regroup(sh, orig, lf, p3);
+  const int64_t gi = i0 + orig;
+  const bool live = gi < a.npts;
+  const int64_t al = live ? gi / a.P : 0;
+  const int am = live ? (int)(gi - al * a.P) : 0;
+  const int dof = conc.dof;
+  AdjBasis bs;
+  bs.build(conc, p3[0], p3[1], p3[2], lf);
+  auto weight = [&](int c) { return live ? ua[(al * dof + c) * a.P + am] : 0.0; };
+  if (A) {
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for_each_wave_leaf(bs.leaf, [&](int leaf) {
+      for (int c = 0; c < dof; ++c) {
+        double gx, gy, gz;
+        grad3(conc.coeff + ((size_t)leaf * dof + c) * NC, bs, gx, gy, gz);
+        const double u = weight(c);
+        ax = __builtin_fma(u, gx, ax);
+        ay = __builtin_fma(u, gy, ay);
+        az = __builtin_fma(u, gz, az);
+      }
+    });
+    if (live) {
+      A[3 * gi + 0] += ax;
+      A[3 * gi + 1] += ay;
+      A[3 * gi + 2] += az;
+    }
+  }
+  scatter_transposed(lam, dof, bs, weight);
+}
+
+// Velocity cotangent: for D = x + tau g2 V(h), h = x + (tau/2) g1 V(x),
+//   a . dD = tau g2 a . dV(h) + tau g2 (tau/2) g1 (J_V(h)^T a) . dV(x),
+// scattered into G[leaf][k][m] at h (in h's leaf) and at x (in x's leaf).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_adj_vel(VelSource vel, TreeView conc, StepArgs a,
+                                                    const double *__restrict__ A,
+                                                    double *__restrict__ G) {
+  const int64_t i0 = (int64_t)blockIdx.x * kBlock;
+  const int64_t gi = i0 + threadIdx.x;
+  const bool live = gi < a.npts;
+  const int64_t i = live ? gi : a.npts - 1;  // idle lanes follow the last point
+  const TreeView &V = vel.t[0];
+  double x, y, z;
+  arrival_point(conc, a, i, x, y, z);
+  if (a.bc == TBX_BC_PERIODIC) {
+    x = wrap1(x);
+    y = wrap1(y);
+    z = wrap1(z);
+  }
+  // the forward's midpoint (rk2_substep, stage 1)
+  double v0, v1, v2;
+  double xs = x, ys = y, zs = z;
+  eval_velocity<D, MODE, kVelTree>(vel, 1, a.ts[0], a.scaled, a.g[0], a.bc, xs, ys, zs, v0, v1,
+                                   v2);
+  double hx = x + a.htau * v0, hy = y + a.htau * v1, hz = z + a.htau * v2;
+  if (a.bc == TBX_BC_PERIODIC) {
+    hx = wrap1(hx);
+    hy = wrap1(hy);
+    hz = wrap1(hz);
+  }
+  const double ax = live ? A[3 * i + 0] : 0.0;
+  const double ay = live ? A[3 * i + 1] : 0.0;
+  const double az = live ? A[3 * i + 2] : 0.0;
+  const double g1 = a.scaled ? a.g[0] : 1.0, g2 = a.scaled ? a.g[1] : 1.0;
+  AdjBasis bh;
+  bh.build(V, hx, hy, hz);
+  // J^T a at h: (J^T a)_k = sum_c a_c dV_c/dy_k
+  double jx = 0.0, jy = 0.0, jz = 0.0;
+  for_each_wave_leaf(bh.leaf, [&](int leaf) {
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+      double gx, gy, gz;
+      grad3(V.coeff + ((size_t)leaf * 3 + c) * NC, bh, gx, gy, gz);
+      const double ac = c == 0 ? ax : c == 1 ? ay : az;
+      jx = __builtin_fma(ac, gx, jx);
+      jy = __builtin_fma(ac, gy, jy);
+      jz = __builtin_fma(ac, gz, jz);
+    }
+  });
+  const double fh = a.tau * g2, fx = a.tau * g2 * a.htau * g1;
+  scatter_transposed(G, 3, bh, [&](int c) {
+    return fh * (c == 0 ? ax : c == 1 ? ay : az);
+  });
+  AdjBasis bx;
+  bx.build(V, x, y, z);
+  scatter_transposed(G, 3, bx, [&](int c) {
+    return fx * (c == 0 ? jx : c == 1 ? jy : jz);
+  });
+}
+
+}  // namespace
+
+#define TBX_CAT2(a, b) a##b
+#define TBX_CAT(a, b) TBX_CAT2(a, b)
+
+hipError_t TBX_CAT(launch_adj_conc_q, TBX_Q)(int mode, const VelSource &vel,
+                                             const TreeView &conc, const StepArgs &a,
+                                             const double *ua, double *lam, double *A,
+                                             hipStream_t s) {
+  if (a.npts <= 0) return hipSuccess;
+  const dim3 g((unsigned)(8 * a.per_xcd)), blk(kAdjBlock);
+  if (mode == kFma64)
+    hipLaunchKernelGGL((k_adj_conc<kFma64, kAdjBlock>), g, blk, 0, s, vel, conc, a, ua, lam, A);
+  else if (mode == kExact64)
+    hipLaunchKernelGGL((k_adj_conc<kExact64, kAdjBlock>), g, blk, 0, s, vel, conc, a, ua, lam,
+                       A);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t TBX_CAT(launch_adj_vel_q, TBX_Q)(int mode, const VelSource &vel,
+                                            const TreeView &conc, const StepArgs &a,
+                                            const double *A, double *G, hipStream_t s) {
+  if (a.npts <= 0) return hipSuccess;
+  const dim3 g((unsigned)((a.npts + kBlock - 1) / kBlock)), blk(kBlock);
+  if (mode == kFma64)
+    hipLaunchKernelGGL((k_adj_vel<kFma64>), g, blk, 0, s, vel, conc, a, A, G);
+  else if (mode == kExact64)
+    hipLaunchKernelGGL((k_adj_vel<kExact64>), g, blk, 0, s, vel, conc, a, A, G);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+}  // namespace tbx

