@@ -202,6 +202,40 @@ int tbx_semilag_insitu_v(const tbx_velocity *vel, tbx_tree *conc, int timestep,
                          double dt, int num_rk_step, int bc, uint32_t flags,
                          double *dep, tbx_stream s);
 
+/* ---- leaves with regular sample grids (tricubic) ----------------------
+ * The reference's fast_interp (tree_functor.h:89-153: 4-point Lagrange per
+ * axis, first index (int)(u (N-1)) - 1 clamped to [0, N-4], zero outside
+ * [0,1]^3; it has no caller there) applied per leaf: each leaf holds an
+ * N^3 grid of samples at corner + 2^-depth (a, b, c) / (N - 1), x fastest,
+ * samples [L][dof][N^3], N in 4..16, evaluated in the leaf's unit
+ * coordinates u = (x - corner) 2^depth.  Flags: TBX_EXACT (fast_interp's
+ * operation order), TBX_FP64, TBX_FP32 (FP32 samples and sums). */
+typedef struct tbx_grid_s tbx_grid;
+typedef struct {
+  int64_t num_leaves;
+  const uint32_t *anchor; /* [L][3], Morton order as tbx_tree_desc */
+  const uint8_t *depth;
+  int32_t n;              /* samples per axis */
+  int32_t dof;
+  const double *samples;  /* host [L][dof][N^3], or NULL for zeros */
+} tbx_grid_desc;
+
+int tbx_grid_create(const tbx_grid_desc *desc, tbx_grid **out);
+int tbx_grid_destroy(tbx_grid *g);
+int tbx_grid_set_samples(tbx_grid *g, const double *samples, int on_device, tbx_stream s);
+int tbx_grid_get_samples(const tbx_grid *g, double *samples, int on_device, tbx_stream s);
+/* Values at pts (device [n][3], wrapped in place with TBX_WRITE_BACK_WRAP
+ * when periodic), out device [n][dof]. */
+int tbx_grid_eval(const tbx_grid *g, double *pts, int64_t n, int bc, uint32_t flags,
+                  double *out, tbx_stream s);
+/* SolveSemilagInSitu for grid leaves (vel: dof 3, scaled by g(t) when g
+ * is given): the sample points' departure points by num_rk_step RK2
+ * substeps, the new samples the values there (no refit); dep (device
+ * [L*N^3][3]) optional. */
+int tbx_grid_semilag_insitu(const tbx_grid *vel, tbx_tfac_fn g, void *gctx, tbx_grid *conc,
+                            int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
+                            double *dep, tbx_stream s);
+
 /* ---- adjoint (no counterpart in the reference) --------------------------
  * The step c' = R E_D(V) c of tbx_semilag_insitu (one RK2 substep, a single
  * velocity tree, optionally scaled by g) is linear in c; its cotangents are

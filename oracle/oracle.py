@@ -44,6 +44,17 @@ class OrcVelocity(C.Structure):
 
 VEL_TREE, VEL_EXTRAP, VEL_SET = 0, 1, 2
 
+
+class OrcGrid(C.Structure):
+    _fields_ = [
+        ("num_leaves", C.c_int64),
+        ("anchor", C.c_void_p),
+        ("depth", C.c_void_p),
+        ("n", C.c_int32),
+        ("dof", C.c_int32),
+        ("samples", C.c_void_p),
+    ]
+
 _lib = None
 
 
@@ -75,6 +86,12 @@ def lib():
             "orc_cubic1d": (C.c_double, [C.c_double, P, P]),
             "orc_field_set_eval": (C.c_int, [P, P, P, C.c_int64, C.c_double, C.c_int, P]),
             "orc_field_extrap_eval": (C.c_int, [P, P, P, C.c_int64, C.c_int, P]),
+            "orc_grid_eval": (C.c_int, [P, P, C.c_int64, C.c_int, P]),
+            "orc_grid_points": (C.c_int, [P, P]),
+            "orc_grid_traj_rk2": (C.c_int, [P, TFAC_FN, P, P, C.c_int64, C.c_double, C.c_double,
+                                            C.c_int, C.c_int, P]),
+            "orc_grid_semilag_insitu": (C.c_int, [P, TFAC_FN, P, P, C.c_int, C.c_double,
+                                                  C.c_int, C.c_int]),
             "orc_traj_rk2_v": (C.c_int, [P, P, C.c_int64, C.c_double, C.c_double, C.c_int,
                                          C.c_int, P]),
             "orc_semilag_insitu_v": (C.c_int, [P, P, P, C.c_int, C.c_double, C.c_int,
@@ -273,3 +290,52 @@ def traj_rk2_v(vel, xinit, tinit, tfinal, num_rk_step, bc=BC_FREESPACE):
 def semilag_insitu_v(vel, conc, M, timestep, dt, num_rk_step=1, bc=BC_FREESPACE):
     _check(lib().orc_semilag_insitu_v(C.byref(vel), conc.ref, _p(np.ascontiguousarray(M)),
                                       timestep, dt, num_rk_step, bc), "semilag_insitu_v")
+
+
+class Grid:
+    """Leaves with regular n^3 sample grids, samples [L][dof][n^3]."""
+
+    def __init__(self, anchor, depth, n, dof, samples=None):
+        self.anchor = np.ascontiguousarray(anchor, dtype=np.uint32).reshape(-1, 3)
+        self.depth = np.ascontiguousarray(depth, dtype=np.uint8).reshape(-1)
+        self.n, self.dof = int(n), int(dof)
+        self.L = len(self.depth)
+        self.P = self.n ** 3
+        if samples is None:
+            samples = np.zeros((self.L, self.dof, self.P))
+        self.samples = np.ascontiguousarray(samples, dtype=np.float64).reshape(
+            self.L, self.dof, self.P).copy()
+        self.s = OrcGrid(self.L, _p(self.anchor), _p(self.depth), self.n, self.dof,
+                         _p(self.samples))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def grid_eval(grid, pts, bc=BC_FREESPACE):
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3).copy()
+    out = np.empty((len(p), grid.dof))
+    _check(lib().orc_grid_eval(grid.ref, _p(p), len(p), bc, _p(out)), "grid_eval")
+    return out
+
+
+def grid_points(grid):
+    out = np.empty((grid.L * grid.P, 3))
+    _check(lib().orc_grid_points(grid.ref, _p(out)), "grid_points")
+    return out
+
+
+def grid_traj_rk2(vel, xinit, tinit, tfinal, num_rk_step, bc=BC_FREESPACE, g=None):
+    x = np.ascontiguousarray(xinit, dtype=np.float64).reshape(-1, 3)
+    out = np.empty_like(x)
+    cb = _tfac(g)
+    _check(lib().orc_grid_traj_rk2(vel.ref, cb, None, _p(x), len(x), tinit, tfinal, num_rk_step,
+                                   bc, _p(out)), "grid_traj_rk2")
+    return out
+
+
+def grid_semilag_insitu(vel, conc, timestep, dt, num_rk_step=1, bc=BC_FREESPACE, g=None):
+    cb = _tfac(g)
+    _check(lib().orc_grid_semilag_insitu(vel.ref, cb, None, conc.ref, timestep, dt, num_rk_step,
+                                         bc), "grid_semilag_insitu")

@@ -734,9 +734,10 @@ int orc_cheb_approx(int q, int dof, const double *vals, double *coeff) {
 /* Tricubic regular-grid interpolation (fast_interp)                         */
 /* ------------------------------------------------------------------------ */
 
-int orc_fast_interp(const double *grid, int dof, int N, const double *pts,
-                    int64_t n, double *out) {
-  if (N < 4 || dof <= 0 || n < 0) return ORC_EINVAL;
+/* fast_interp for one query point p in [0,1]^3 of one regular N^3 grid
+ * (tree_functor.h:89-153); out[dof]. */
+static void fast_interp_point(const double *grid, int dof, int N, const double *p,
+                              double *out) {
   double lagr_denom[4];
   for (int i = 0; i < 4; i++) { /* tree_functor.h:93-99 */
     lagr_denom[i] = 1;
@@ -744,47 +745,175 @@ int orc_fast_interp(const double *grid, int dof, int N, const double *pts,
       if (i != j) lagr_denom[i] /= (double)(i - j);
   }
   const int64_t N3 = (int64_t)N * N * N;
+  if (p[0] < 0 || p[0] > 1.0 || p[1] < 0 || p[1] > 1.0 || p[2] < 0 ||
+      p[2] > 1.0) { /* :106-114 */
+    for (int k = 0; k < dof; k++) out[k] = 0;
+    return;
+  }
+  double point[3];
+  int gi[3];
+  for (int j = 0; j < 3; j++) { /* :118-124 */
+    point[j] = p[j] * (N - 1);
+    gi[j] = ((int)point[j]) - 1;
+    if (gi[j] < 0) gi[j] = 0;
+    if (gi[j] > N - 4) gi[j] = N - 4;
+    point[j] -= gi[j];
+  }
+  double Mw[3][4];
+  for (int j = 0; j < 3; j++) { /* :127-135 */
+    double x = point[j];
+    for (int k = 0; k < 4; k++) {
+      Mw[j][k] = lagr_denom[k];
+      for (int l = 0; l < 4; l++)
+        if (k != l) Mw[j][k] *= (x - l);
+    }
+  }
+  for (int k = 0; k < dof; k++) { /* :137-151 */
+    double val = 0;
+    for (int j2 = 0; j2 < 4; j2++)
+      for (int j1 = 0; j1 < 4; j1++) {
+        double M1M2 = Mw[1][j1] * Mw[2][j2];
+        int64_t ij = (int64_t)N * ((gi[1] + j1) + (int64_t)N * (gi[2] + j2));
+        for (int j0 = 0; j0 < 4; j0++) {
+          int64_t idx = (gi[0] + j0) + ij;
+          val += Mw[0][j0] * M1M2 * grid[idx + k * N3];
+        }
+      }
+    out[k] = val;
+  }
+}
+
+int orc_fast_interp(const double *grid, int dof, int N, const double *pts,
+                    int64_t n, double *out) {
+  if (N < 4 || dof <= 0 || n < 0) return ORC_EINVAL;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) fast_interp_point(grid, dof, N, pts + 3 * i, out + i * dof);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Regular-grid leaves (tricubic)                                            */
+/* ------------------------------------------------------------------------ */
+
+static orc_tree grid_as_tree(const orc_grid *g) {
+  orc_tree t = {g->num_leaves, g->anchor, g->depth, 1, g->dof, NULL};
+  return t;
+}
+
+int orc_grid_eval(const orc_grid *g, double *pts, int64_t n, int bc, double *out) {
+  if (!g || g->n < 4 || g->dof <= 0 || n < 0) return ORC_EINVAL;
+  if (n == 0) return ORC_OK;
+  orc_tree t = grid_as_tree(g);
+  if (bc == ORC_BC_PERIODIC) wrap_periodic(pts, 3 * n);
+  int32_t *leaf = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  if (!leaf) return ORC_ENOMEM;
+  int rc = orc_locate(&t, pts, n, bc, leaf);
+  if (rc) {
+    free(leaf);
+    return rc;
+  }
+  const int64_t N3 = (int64_t)g->n * g->n * g->n;
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; i++) {
-    const double *p = pts + 3 * i;
-    if (p[0] < 0 || p[0] > 1.0 || p[1] < 0 || p[1] > 1.0 || p[2] < 0 ||
-        p[2] > 1.0) { /* :106-114 */
-      for (int k = 0; k < dof; k++) out[i * dof + k] = 0;
-      continue;
-    }
-    double point[3];
-    int gi[3];
-    for (int j = 0; j < 3; j++) { /* :118-124 */
-      point[j] = p[j] * (N - 1);
-      gi[j] = ((int)point[j]) - 1;
-      if (gi[j] < 0) gi[j] = 0;
-      if (gi[j] > N - 4) gi[j] = N - 4;
-      point[j] -= gi[j];
-    }
-    double Mw[3][4];
-    for (int j = 0; j < 3; j++) { /* :127-135 */
-      double x = point[j];
-      for (int k = 0; k < 4; k++) {
-        Mw[j][k] = lagr_denom[k];
-        for (int l = 0; l < 4; l++)
-          if (k != l) Mw[j][k] *= (x - l);
-      }
-    }
-    for (int k = 0; k < dof; k++) { /* :137-151 */
-      double val = 0;
-      for (int j2 = 0; j2 < 4; j2++)
-        for (int j1 = 0; j1 < 4; j1++) {
-          double M1M2 = Mw[1][j1] * Mw[2][j2];
-          int64_t ij = (int64_t)N * ((gi[1] + j1) + (int64_t)N * (gi[2] + j2));
-          for (int j0 = 0; j0 < 4; j0++) {
-            int64_t idx = (gi[0] + j0) + ij;
-            val += Mw[0][j0] * M1M2 * grid[idx + k * N3];
-          }
-        }
-      out[i * dof + k] = val;
+    const int64_t j = leaf[i];
+    const double s = ldexp(1.0, g->depth[j]), len = 1.0 / s;
+    double u[3];
+    for (int a = 0; a < 3; a++) u[a] = (pts[3 * i + a] - g->anchor[3 * j + a] * len) * s;
+    fast_interp_point(g->samples + j * g->dof * N3, g->dof, g->n, u, out + i * g->dof);
+  }
+  free(leaf);
+  return ORC_OK;
+}
+
+int orc_grid_points(const orc_grid *g, double *out) {
+  if (!g || g->n < 2) return ORC_EINVAL;
+  const int N = g->n;
+  const int64_t N3 = (int64_t)N * N * N;
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < g->num_leaves; j++) {
+    const double len = 1.0 / ldexp(1.0, g->depth[j]);
+    const double cx = g->anchor[3 * j] * len, cy = g->anchor[3 * j + 1] * len,
+                 cz = g->anchor[3 * j + 2] * len;
+    for (int64_t m = 0; m < N3; m++) {
+      const int a = (int)(m % N), b = (int)((m / N) % N), c = (int)(m / ((int64_t)N * N));
+      double *o = out + 3 * (j * N3 + m);
+      o[0] = cx + len * ((double)a / (N - 1));
+      o[1] = cy + len * ((double)b / (N - 1));
+      o[2] = cz + len * ((double)c / (N - 1));
     }
   }
   return ORC_OK;
+}
+
+static int grid_vel(const orc_grid *v, orc_tfac_fn g, void *gctx, double *pts, int64_t n,
+                    double t, int bc, double *out) {
+  int rc = orc_grid_eval(v, pts, n, bc, out);
+  if (rc || !g) return rc;
+  const double f = g(t, gctx);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < 3 * n; i++) out[i] = out[i] * f;
+  return ORC_OK;
+}
+
+int orc_grid_traj_rk2(const orc_grid *vel, orc_tfac_fn g, void *gctx, const double *xinit,
+                      int64_t n, double tinit, double tfinal, int num_rk_step, int bc,
+                      double *xsol) {
+  if (!vel || vel->dof != 3 || num_rk_step <= 0 || n < 0) return ORC_EINVAL;
+  if (n == 0) return ORC_OK;
+  const int64_t m = 3 * n;
+  const double tau = (tfinal - tinit) / num_rk_step;
+  if (xsol != xinit) memcpy(xsol, xinit, sizeof(double) * (size_t)m);
+  double *xtmp = (double *)malloc(sizeof(double) * (size_t)m);
+  double *vtmp = (double *)malloc(sizeof(double) * (size_t)m);
+  if (!xtmp || !vtmp) {
+    free(xtmp);
+    free(vtmp);
+    return ORC_ENOMEM;
+  }
+  double tcur = tinit;
+  int rc = ORC_OK;
+  for (int s = 0; s < num_rk_step && !rc; s++) { /* traj.inc:20-68 */
+    rc = grid_vel(vel, g, gctx, xsol, n, tcur, bc, vtmp);
+    if (rc) break;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; i++) xtmp[i] = xsol[i] + 0.5 * tau * vtmp[i];
+    rc = grid_vel(vel, g, gctx, xtmp, n, tcur + 0.5 * tau, bc, vtmp);
+    if (rc) break;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; i++) xsol[i] = xsol[i] + tau * vtmp[i];
+    tcur = tcur + tau;
+  }
+  free(xtmp);
+  free(vtmp);
+  return rc;
+}
+
+int orc_grid_semilag_insitu(const orc_grid *vel, orc_tfac_fn g, void *gctx, orc_grid *conc,
+                            int timestep, double dt, int num_rk_step, int bc) {
+  if (!vel || !conc || conc->n < 4) return ORC_EINVAL;
+  const int dof = conc->dof;
+  const int64_t N3 = (int64_t)conc->n * conc->n * conc->n, n = conc->num_leaves * N3;
+  double *pts = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double *vals = (double *)malloc(sizeof(double) * dof * (size_t)n);
+  if (!pts || !vals) {
+    free(pts);
+    free(vals);
+    return ORC_ENOMEM;
+  }
+  int rc = orc_grid_points(conc, pts);
+  const double tinit = timestep * dt, tfinal = tinit - dt; /* semilag.inc:34-35 */
+  if (!rc) rc = orc_grid_traj_rk2(vel, g, gctx, pts, n, tinit, tfinal, num_rk_step, bc, pts);
+  if (!rc) rc = orc_grid_eval(conc, pts, n, bc, vals);
+  if (!rc) {
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < conc->num_leaves; j++)
+      for (int l = 0; l < dof; l++)
+        for (int64_t p = 0; p < N3; p++)
+          conc->samples[(j * dof + l) * N3 + p] = vals[(j * N3 + p) * dof + l];
+  }
+  free(pts);
+  free(vals);
+  return rc;
 }
 
 /* ------------------------------------------------------------------------ */

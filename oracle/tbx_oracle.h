@@ -23,6 +23,8 @@
  *   orc_cheb_approx      polynomial reproduction, numpy DCT
  *   orc_fast_interp      exact for per-axis cubics, 0 outside [0,1]^3
  *   orc_cubic1d          exact for quadratics on uniform times
+ *   orc_grid_eval        per-axis cubics reproduced, leaf-local coordinates
+ *                        against locate + orc_fast_interp on the leaf grid
  * The PVFMM routines the reference calls (MortonId, cheb_poly, cheb_eval,
  * Matrix::pinv, cheb_approx) are not in the reference repository; their
  * semantics are defined here as documented per function and pinned by the
@@ -113,6 +115,30 @@ int orc_cheb_approx(int q, int dof, const double *vals, double *coeff);
  * N^3 grid of [0,1]^3, values [dof][N^3], x fastest. */
 int orc_fast_interp(const double *grid, int dof, int N, const double *pts,
                     int64_t n, double *out);
+
+/* Leaves holding regular N^3 sample grids (values at corner + 2^-depth *
+ * (a, b, c) / (N - 1), x fastest, [L][dof][N^3]) evaluated by fast_interp
+ * in the leaf's unit coordinates u = (x - corner) * 2^depth.  The
+ * reference's fast_interp (tree_functor.h:89-153) has no caller there; this
+ * tree-level use defines the tricubic variant of the step. */
+typedef struct {
+  int64_t num_leaves;
+  const uint32_t *anchor;
+  const uint8_t *depth;
+  int32_t n;
+  int32_t dof;
+  double *samples;
+} orc_grid;
+int orc_grid_eval(const orc_grid *g, double *pts, int64_t n, int bc, double *out);
+int orc_grid_points(const orc_grid *g, double *out);
+int orc_grid_traj_rk2(const orc_grid *vel, orc_tfac_fn g, void *gctx,
+                      const double *xinit, int64_t n, double tinit, double tfinal,
+                      int num_rk_step, int bc, double *xsol);
+/* SolveSemilagInSitu with tricubic leaves: the new samples are the values
+ * at the departure points of the sample points (no refit). */
+int orc_grid_semilag_insitu(const orc_grid *vel, orc_tfac_fn g, void *gctx,
+                            orc_grid *conc, int timestep, double dt,
+                            int num_rk_step, int bc);
 
 /* CubicInterpPolicy::InterpCubic1D (utils/cubic.h:42-56). */
 double orc_cubic1d(double x, const double *xx, const double *pp);

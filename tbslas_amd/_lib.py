@@ -48,6 +48,17 @@ class _TreeDesc(C.Structure):
     ]
 
 
+class _GridDesc(C.Structure):
+    _fields_ = [
+        ("num_leaves", C.c_int64),
+        ("anchor", C.c_void_p),
+        ("depth", C.c_void_p),
+        ("n", C.c_int32),
+        ("dof", C.c_int32),
+        ("samples", C.c_void_p),
+    ]
+
+
 class _Velocity(C.Structure):
     _fields_ = [
         ("kind", C.c_int32),
@@ -92,6 +103,12 @@ _SIGS = {
                                       _P, _P]),
     "tbx_adjoint_run": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _U32, _P, _I32, _P, _P,
                                C.POINTER(C.c_double), _P]),
+    "tbx_grid_create": (_I32, [C.POINTER(_GridDesc), C.POINTER(_P)]),
+    "tbx_grid_destroy": (_I32, [_P]),
+    "tbx_grid_set_samples": (_I32, [_P, _P, _I32, _P]),
+    "tbx_grid_get_samples": (_I32, [_P, _P, _I32, _P]),
+    "tbx_grid_eval": (_I32, [_P, _P, _I64, _I32, _U32, _P, _P]),
+    "tbx_grid_semilag_insitu": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P]),
     "tbx_new_nodes": (_I32, [_I32, _P]),
     "tbx_pt2coeff_matrix": (_I32, [_I32, _P]),
     "tbx_uniform_leaves": (_I32, [_I32, _P, _P]),

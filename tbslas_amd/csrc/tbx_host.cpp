@@ -1062,6 +1062,167 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
   return rc;
 }
 
+struct tbx_grid_s {
+  tbx_tree *tree = nullptr;  // leaf structure (keys, directory); its coefficients are unused
+  int n = 0, dof = 0;
+  int64_t L = 0;
+  double *d_s = nullptr, *d_s_next = nullptr;
+  float *d_s32 = nullptr;
+  bool s32_stale = true;
+  int64_t nsamples() const { return L * dof * (int64_t)n * n * n; }
+};
+
+namespace {
+
+void free_grid(tbx_grid *g) {
+  if (!g) return;
+  if (g->tree) tbx_tree_destroy(g->tree);
+  hipFree(g->d_s);
+  hipFree(g->d_s_next);
+  hipFree(g->d_s32);
+  delete g;
+}
+
+int grid_mode(uint32_t flags, int *mode, const char *who) {
+  int rc = mode_of(flags, mode);
+  if (rc) return rc;
+  if (*mode == kFp32h) return fail(TBX_EUNSUPPORTED, "%s: no FP64-head mode for grid leaves", who);
+  return TBX_OK;
+}
+
+int grid_prepare(tbx_grid *g, int mode, hipStream_t st) {
+  if (mode != kFp32) return TBX_OK;
+  if (!g->d_s32) {
+    HIP_TRY(hipMalloc(&g->d_s32, sizeof(float) * (size_t)g->nsamples()));
+    g->s32_stale = true;
+  }
+  if (g->s32_stale) {
+    HIP_TRY(launch_to_f32(g->d_s, g->d_s32, g->nsamples(), st));
+    g->s32_stale = false;
+  }
+  return TBX_OK;
+}
+
+}  // namespace
+
+int tbx_grid_create(const tbx_grid_desc *desc, tbx_grid **out) {
+  if (!desc || !out) return fail(TBX_EINVAL, "tbx_grid_create: null argument");
+  *out = nullptr;
+  if (desc->n < 4 || desc->n > 16)
+    return fail(TBX_EUNSUPPORTED, "tbx_grid_create: %d samples per axis outside 4..16", desc->n);
+  if (desc->dof < 1) return fail(TBX_EINVAL, "tbx_grid_create: dof %d", desc->dof);
+  tbx_tree_desc td{desc->num_leaves, desc->anchor, desc->depth, 1, 1, nullptr};
+  tbx_grid *g = new tbx_grid_s();
+  int rc = tbx_tree_create(&td, &g->tree);
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  g->n = desc->n;
+  g->dof = desc->dof;
+  g->L = desc->num_leaves;
+  if ((double)g->nsamples() > 4.0e10) {
+    free_grid(g);
+    return fail(TBX_EOVERFLOW, "tbx_grid_create: sample array too large");
+  }
+  const size_t bytes = sizeof(double) * (size_t)g->nsamples();
+  hipError_t e;
+  if ((e = hipMalloc(&g->d_s, bytes)) != hipSuccess ||
+      (e = hipMalloc(&g->d_s_next, bytes)) != hipSuccess) {
+    free_grid(g);
+    return fail(TBX_ENOMEM, "tbx_grid_create: hipMalloc: %s", hipGetErrorString(e));
+  }
+  e = desc->samples ? hipMemcpy(g->d_s, desc->samples, bytes, hipMemcpyHostToDevice)
+                    : hipMemset(g->d_s, 0, bytes);
+  if (e != hipSuccess) {
+    free_grid(g);
+    return fail(TBX_EHIP, "tbx_grid_create: upload: %s", hipGetErrorString(e));
+  }
+  *out = g;
+  return TBX_OK;
+}
+
+int tbx_grid_destroy(tbx_grid *g) {
+  if (!g) return fail(TBX_EINVAL, "tbx_grid_destroy: null grid");
+  hipDeviceSynchronize();
+  free_grid(g);
+  return TBX_OK;
+}
+
+int tbx_grid_set_samples(tbx_grid *g, const double *samples, int on_device, tbx_stream s) {
+  if (!g || !samples) return fail(TBX_EINVAL, "tbx_grid_set_samples: null argument");
+  HIP_TRY(hipMemcpyAsync(g->d_s, samples, sizeof(double) * (size_t)g->nsamples(),
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                         (hipStream_t)s));
+  if (!on_device) HIP_TRY(hipStreamSynchronize((hipStream_t)s));
+  g->s32_stale = true;
+  return TBX_OK;
+}
+
+int tbx_grid_get_samples(const tbx_grid *g, double *samples, int on_device, tbx_stream s) {
+  if (!g || !samples) return fail(TBX_EINVAL, "tbx_grid_get_samples: null argument");
+  HIP_TRY(hipMemcpyAsync(samples, g->d_s, sizeof(double) * (size_t)g->nsamples(),
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                         (hipStream_t)s));
+  if (!on_device) HIP_TRY(hipStreamSynchronize((hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_grid_eval(const tbx_grid *gc, double *pts, int64_t n, int bc, uint32_t flags,
+                  double *out, tbx_stream s) {
+  tbx_grid *g = const_cast<tbx_grid *>(gc);
+  if (!g) return fail(TBX_EINVAL, "tbx_grid_eval: null grid");
+  int rc = check_points(pts, n, "tbx_grid_eval");
+  if (rc) return rc;
+  if ((rc = check_bc(bc))) return rc;
+  if (n > 0 && !out) return fail(TBX_EINVAL, "tbx_grid_eval: null output");
+  int mode;
+  if ((rc = grid_mode(flags, &mode, "tbx_grid_eval"))) return rc;
+  if ((rc = grid_prepare(g, mode, (hipStream_t)s))) return rc;
+  HIP_TRY(launch_grid_eval(mode, g->tree->view(), g->d_s, g->d_s32, g->n, g->dof, pts, n, bc,
+                           (flags & TBX_WRITE_BACK_WRAP) ? 1 : 0, out, (hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_grid_semilag_insitu(const tbx_grid *velc, tbx_tfac_fn g, void *gctx, tbx_grid *conc,
+                            int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
+                            double *dep, tbx_stream s) {
+  tbx_grid *vel = const_cast<tbx_grid *>(velc);
+  const char *who = "tbx_grid_semilag_insitu";
+  if (!vel || !conc) return fail(TBX_EINVAL, "%s: null grid", who);
+  if (vel->dof != 3) return fail(TBX_EINVAL, "%s: velocity dof %d", who, vel->dof);
+  if (num_rk_step <= 0 || num_rk_step > kMaxSub)
+    return fail(TBX_EUNSUPPORTED, "%s: num_rk_step %d outside 1..%d", who, num_rk_step, kMaxSub);
+  int rc = check_bc(bc);
+  if (rc) return rc;
+  int mode;
+  if ((rc = grid_mode(flags, &mode, who))) return rc;
+  hipStream_t st = (hipStream_t)s;
+  if ((rc = grid_prepare(vel, mode, st)) || (rc = grid_prepare(conc, mode, st))) return rc;
+  GridStepArgs a{};
+  const int N = conc->n;
+  a.npts = conc->L * (int64_t)N * N * N;
+  a.nsub = num_rk_step;
+  a.bc = bc;
+  a.scaled = g ? 1 : 0;
+  const double tinit = timestep * dt, tfinal = tinit - dt;  // semilag.inc:34-35
+  a.tau = (tfinal - tinit) / num_rk_step;                    // traj.inc:55
+  a.htau = 0.5 * a.tau;
+  double tcur = tinit;
+  for (int k = 0; k < num_rk_step; ++k) {
+    a.g[2 * k] = g ? g(tcur, gctx) : 1.0;
+    a.g[2 * k + 1] = g ? g(tcur + 0.5 * a.tau, gctx) : 1.0;
+    tcur = tcur + a.tau;
+  }
+  for (int j = 0; j < N; ++j) a.node[j] = (double)j / (N - 1);
+  HIP_TRY(launch_grid_step(mode, vel->tree->view(), vel->d_s, vel->d_s32, vel->n,
+                           conc->tree->view(), conc->d_s, conc->d_s32, N, conc->dof, a,
+                           conc->d_s_next, dep, st));
+  std::swap(conc->d_s, conc->d_s_next);
+  conc->s32_stale = true;
+  return TBX_OK;
+}
+
 int tbx_set_option(int option, int value) {
   if (option == TBX_OPT_STEP_SHAPE) {
     if (value < 0 || value >= kStepShapes)

@@ -134,6 +134,26 @@ TBX_DECLARE_Q(14)
 TBX_DECLARE_Q(15)
 TBX_DECLARE_Q(16)
 
+// Fused step of grid (tricubic) leaves.
+struct GridStepArgs {
+  int64_t npts;  // L * N^3 sample points
+  int32_t nsub;
+  int32_t bc;
+  int32_t scaled;
+  double tau, htau;
+  double g[2 * kMaxSub];
+  double node[16];  // a / (N - 1)
+};
+
+// Grid-leaf launchers (tbx_grid.hip).
+hipError_t launch_grid_eval(int mode, const TreeView &t, const double *s, const float *s32,
+                            int n, int dof, double *pts, int64_t npts, int bc, int write_back,
+                            double *out, hipStream_t st);
+hipError_t launch_grid_step(int mode, const TreeView &tv, const double *vs, const float *vs32,
+                            int vn, const TreeView &tc, const double *cs, const float *cs32,
+                            int cn, int cdof, const GridStepArgs &a, double *out, double *dep,
+                            hipStream_t st);
+
 // Degree-independent launchers (tbx_common.hip).
 hipError_t launch_locate(const TreeView &t, const double *pts, int64_t n,
                          int bc, int32_t *leaf, hipStream_t s);

@@ -199,3 +199,49 @@ def test_insitu_with_velocity_kinds(orc, tb):
     orc.semilag_insitu_v(orc.velocity(orc.VEL_EXTRAP, [vel, vel]), b, M, 2, 0.01, 2, 0)
     # 1.5 v - 0.5 v differs from v by rounding only
     assert np.abs(a.coeff - b.coeff).max() <= 1e-13 * np.abs(a.coeff).max()
+
+
+def test_grid_leaves_reproduce_cubics_and_follow_fast_interp(orc, tb):
+    # regular-grid leaves: a per-axis cubic is reproduced exactly in every
+    # leaf, and a leaf's value is fast_interp of its own grid at the leaf's
+    # unit coordinates
+    anchor, depth = tb.uniform_leaves(2)
+    N = 8
+    g0 = orc.Grid(anchor, depth, N, 2)
+    pts = orc.grid_points(g0)
+    f = lambda p: np.stack([(p[:, 0] ** 3 - p[:, 1]) * (1 + p[:, 2] ** 2),
+                            p[:, 0] * p[:, 1] * p[:, 2]], axis=1)
+    vals = f(pts).reshape(len(depth), N ** 3, 2).transpose(0, 2, 1)
+    g = orc.Grid(anchor, depth, N, 2, vals)
+    x = np.random.default_rng(3).random((3000, 3))
+    got = orc.grid_eval(g, x)
+    assert np.abs(got - f(x)).max() <= 1e-13
+    j = orc.locate(orc.Tree(anchor, depth, 1, 1), x)
+    s = 2.0 ** depth[j].astype(float)
+    u = (x - anchor[j] / s[:, None]) * s[:, None]
+    for i in range(0, 3000, 300):
+        ref = orc.fast_interp(g.samples[j[i]], 2, N, u[i:i + 1])
+        assert np.array_equal(got[i:i + 1], ref)
+
+
+def test_grid_step_translation_is_exact_for_cubics(orc, tb):
+    # a constant velocity moves the sample points rigidly; the step's new
+    # samples are the cubic's values at the translated points
+    anchor, depth = tb.uniform_leaves(2)
+    N = 6
+    w = np.array([0.13, -0.07, 0.05])
+    pts = orc.grid_points(orc.Grid(anchor, depth, N, 1))
+    vv = np.broadcast_to(w, pts.shape).reshape(len(depth), N ** 3, 3).transpose(0, 2, 1)
+    vel = orc.Grid(anchor, depth, N, 3, vv)
+    f = lambda p: (p[:, 0] ** 2 * p[:, 1] + p[:, 2] ** 3)[:, None]
+    conc = orc.Grid(anchor, depth, N, 1, f(pts).reshape(len(depth), 1, -1))
+    dt = 0.5
+    orc.grid_semilag_insitu(vel, conc, 1, dt, 1, 0)
+    dep = pts - dt * w
+    mid = pts - 0.5 * dt * w
+    # the velocity is 0 outside the domain (FreeSpace), so only trajectories
+    # whose midpoint and departure point stay inside are rigid translations
+    inside = np.all((dep >= 0) & (dep < 1) & (mid >= 0) & (mid < 1), axis=1)
+    got = conc.samples.reshape(-1)
+    assert inside.sum() > len(inside) // 2
+    assert np.abs(got[inside] - f(dep[inside])[:, 0]).max() <= 1e-12
