@@ -129,6 +129,111 @@ def cpu_baseline(anchor, depth, ccf, vcf, n_leaves: int):
     return 3 * npts / t, npts, n_leaves, t
 
 
+def c5_measure(steps: int):
+    """The tricubic variant at full size: 262,144 leaves (uniform depth 6)
+    of 8^3 samples, Lamb-Oseen pair velocity, smoothed slotted sphere,
+    dt = 5e-4, FreeSpace.  Returns per-mode step times and the achieved HBM
+    fraction on algorithmic bytes (samples read once: 3 velocity + 1
+    concentration component, new samples written once as FP64)."""
+    import torch
+
+    import tbslas_amd as tb
+    from tbslas_amd import fields
+
+    anchor, depth = tb.uniform_leaves(6)
+    n = 8
+    vs = tb.sample_function(fields.lamb_oseen_pair, anchor, depth, n, 3)
+    cs = tb.sample_function(lambda p: fields.smoothed_slotted_sphere(p, 2.0 / (64 * (n - 1))),
+                            anchor, depth, n, 1)
+    vel = tb.Grid(anchor, depth, n, 3, vs)
+    npts = len(depth) * n ** 3
+    out = {"leaves": len(depth), "samples_per_leaf": n ** 3, "departure_points": npts,
+           "dt": 5e-4, "steps": steps}
+    for mode, sb in (("fp64", 8), ("fp32", 4)):
+        conc = tb.Grid(anchor, depth, n, 1, cs)
+        tb.grid_semilag_insitu(vel, conc, 1, 5e-4, 1, 0, mode)  # warm-up
+        conc.set_samples(cs)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(1, steps + 1):
+            tb.grid_semilag_insitu(vel, conc, k, 5e-4, 1, 0, mode)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        nbytes = npts * (4 * sb + 8)
+        out[mode] = {"ms_per_step": ms, "departure_points_per_s": npts / (ms * 1e-3),
+                     "interps_per_s": 3 * npts / (ms * 1e-3),
+                     "algorithmic_bytes": nbytes,
+                     "hbm_gbs": nbytes / (ms * 1e-3) / 1e9,
+                     "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK}
+        del conc
+    return out
+
+
+def c3a_measure(steps: int = 2):
+    """Batched tracers at full size: 64 seeded Gaussian tracers (SplitMix64
+    seed 3) on a uniform depth-6 tree (262,144 leaves), q = 8, carried by one
+    trajectory set of the rigid rotation v = 2 pi (0.5 - y, x - 0.5, 0),
+    dt = 1/200, FreeSpace; a 64-component concentration tree, one in-situ
+    step = trajectories once + 64 contractions per departure point + refit.
+    interps/step = N (2 + 64).  Accuracy: FP32 against FP64 after one step
+    at 2^20 random points."""
+    import torch
+
+    import tbslas_amd as tb
+    from tbslas_amd import fields
+
+    q, T, depth_u = 8, 64, 6
+    anchor, depth = tb.uniform_leaves(depth_u)
+    L, P = len(depth), (q + 1) ** 3
+    N = L * P
+    omega, dt = 2 * math.pi, 1.0 / 200
+    vel = tb.Tree(anchor, depth, q, 3, tb.fit_function(lambda p: fields.rotation(p, omega),
+                                                       anchor, depth, q, 3))
+    centres, sigma = fields.tracer_blobs(T, 3)
+    # initial coefficients on the device: per tracer, values at the grid
+    # points and the refit of a one-component tree
+    one = tb.Tree(anchor, depth, q, 1)
+    gp = one.grid_points()
+    C = torch.empty((L, T, tb.ncoef(q)), dtype=torch.float64, device="cuda")
+    ct = torch.from_numpy(centres).cuda()
+    st = torch.from_numpy(sigma).cuda()
+    for k in range(T):
+        v = torch.exp(-((gp - ct[k]) ** 2).sum(1) / (2 * st[k] ** 2))
+        one.set_grid_values(v)
+        C[:, k, :] = one.get_coeffs_device()[:, 0, :]
+        del v
+    del gp, one
+    torch.cuda.empty_cache()
+    conc = tb.Tree(anchor, depth, q, T)
+    probe = torch.rand((1 << 20, 3), dtype=torch.float64, device="cuda",
+                       generator=torch.Generator(device="cuda").manual_seed(3))
+    out = {"tracers": T, "leaves": L, "q": q, "departure_points": N,
+           "interps_per_step": N * (2 + T), "dt": dt, "steps": steps}
+    res = {}
+    for mode in ("fp64", "fp32"):
+        conc.set_coeffs(C)
+        tb.semilag_insitu(vel, conc, 1, dt, 1, 0, mode)  # warm-up, also the accuracy state
+        res[mode] = conc.eval(probe.clone(), 0, "fp64")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(2, steps + 2):
+            tb.semilag_insitu(vel, conc, k, dt, 1, 0, mode)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out[mode] = {"ms_per_step": ms, "interps_per_s": N * (2 + T) / (ms * 1e-3),
+                     "departure_points_per_s": N / (ms * 1e-3)}
+    out["linf_rel_fp32_vs_fp64_after_1_step"] = float(
+        (res["fp32"] - res["fp64"]).abs().max() / res["fp64"].abs().max())
+    out["fp32_over_fp64"] = out["fp32"]["interps_per_s"] / out["fp64"]["interps_per_s"]
+    del conc, C, vel, probe, res
+    torch.cuda.empty_cache()
+    return out
+
+
 def traffic_from_profiles(kernel: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
     summary (profiles/step_traffic.txt: "kernel bytes" lines), if measured."""
@@ -156,6 +261,8 @@ def main():
                          "for byte counters), then --steps step kernels of one "
                          "arithmetic mode (fp64, fp32, fp32h, exact); no JSON line")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c5", action="store_true", help="skip the tricubic-variant measurement")
+    ap.add_argument("--no-c3a", action="store_true", help="skip the 64-tracer measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
@@ -280,6 +387,13 @@ def main():
         "roofline_fp32": roof("fp32"),
         "roofline_fp64": roof("fp64"),
     }
+    if not args.no_c5:
+        del trees, vel
+        torch.cuda.empty_cache()
+        out["c5_tricubic"] = c5_measure(10)
+    if not args.no_c3a:
+        torch.cuda.empty_cache()
+        out["c3a_64_tracers"] = c3a_measure(2)
     if not args.no_cpu:
         v, npts, nl, t = cpu_baseline(anchor, depth, ccf, vcf, args.cpu_leaves)
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

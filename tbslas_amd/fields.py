@@ -171,3 +171,19 @@ def random_poly_velocity(q: int, seed: int = 4):
         return out
 
     return f
+
+
+def tracer_blobs(n: int = 64, seed: int = 3):
+    """n Gaussian tracer blobs: centres uniform in [0.3, 0.7]^3 and sigma
+    uniform in [0.03, 0.08], drawn from SplitMix64(seed) in the order
+    (cx, cy, cz, sigma) per tracer.  Returns (centres [n,3], sigma [n])."""
+    u = splitmix64(seed, 4 * n).reshape(n, 4)
+    return 0.3 + 0.4 * u[:, :3], 0.03 + 0.05 * u[:, 3]
+
+
+def gaussian_blobs(pts, centres, sigma):
+    """Values [m, n] of n Gaussian blobs exp(-|x - c|^2 / (2 sigma^2)) at pts
+    [m, 3]; works on numpy arrays and torch tensors alike."""
+    d2 = ((pts[:, None, :] - centres[None, :, :]) ** 2).sum(-1)
+    return (-d2 / (2 * sigma[None, :] ** 2)).exp() if hasattr(d2, "exp") else np.exp(
+        -d2 / (2 * sigma[None, :] ** 2))
