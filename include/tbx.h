@@ -236,6 +236,31 @@ int tbx_grid_semilag_insitu(const tbx_grid *vel, tbx_tfac_fn g, void *gctx, tbx_
                             int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
                             double *dep, tbx_stream s);
 
+/* ---- many small problems in one persistent launch --------------------
+ * A batch advances nprob independent problems (velocity tree vel[p], dof 3;
+ * concentration tree conc[p], dof 1; degree 6, 8 or 10; time step dt[p];
+ * stationary velocity, one RK2 substep) by one in-situ step each in a
+ * single persistent kernel, in FP32: work items of a few leaves are claimed
+ * from a global counter, each item evaluates its departure points and
+ * refits its leaves at once.  The two steps of each parity are captured as
+ * HIP graphs at configuration time and replayed by tbx_batch_step.  The
+ * batch keeps its own FP32 coefficients (initialised from conc[p]); the
+ * trees provide the leaf structure and must outlive the batch.  The
+ * configuration (workgroup size x leaves per item) is chosen by
+ * tbx_batch_tune, which times every one on the current state.  No
+ * counterpart in the reference, whose ranks each own one tree. */
+typedef struct tbx_batch_s tbx_batch;
+int tbx_batch_create(int nprob, const tbx_tree *const *vel, const tbx_tree *const *conc,
+                     const double *dt, int bc, tbx_batch **out);
+int tbx_batch_destroy(tbx_batch *b);
+int tbx_batch_set_config(tbx_batch *b, int cfg);       /* 0..5 */
+int tbx_batch_get_config(const tbx_batch *b, int *cfg);
+/* Times each configuration (ms[6], may be NULL), keeps the fastest. */
+int tbx_batch_tune(tbx_batch *b, double *ms, tbx_stream s);
+int tbx_batch_step(tbx_batch *b, int nsteps, tbx_stream s);
+/* Current coefficients of problem p, host [L][1][Nc] (FP32 state). */
+int tbx_batch_get_coeffs(const tbx_batch *b, int p, double *coeff, tbx_stream s);
+
 /* ---- adjoint (no counterpart in the reference) --------------------------
  * The step c' = R E_D(V) c of tbx_semilag_insitu (one RK2 substep, a single
  * velocity tree, optionally scaled by g) is linear in c; its cotangents are

@@ -109,6 +109,13 @@ _SIGS = {
     "tbx_grid_get_samples": (_I32, [_P, _P, _I32, _P]),
     "tbx_grid_eval": (_I32, [_P, _P, _I64, _I32, _U32, _P, _P]),
     "tbx_grid_semilag_insitu": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P]),
+    "tbx_batch_create": (_I32, [_I32, _P, _P, _P, _I32, C.POINTER(_P)]),
+    "tbx_batch_destroy": (_I32, [_P]),
+    "tbx_batch_set_config": (_I32, [_P, _I32]),
+    "tbx_batch_get_config": (_I32, [_P, C.POINTER(_I32)]),
+    "tbx_batch_tune": (_I32, [_P, _P, _P]),
+    "tbx_batch_step": (_I32, [_P, _I32, _P]),
+    "tbx_batch_get_coeffs": (_I32, [_P, _I32, _P, _P]),
     "tbx_new_nodes": (_I32, [_I32, _P]),
     "tbx_pt2coeff_matrix": (_I32, [_I32, _P]),
     "tbx_uniform_leaves": (_I32, [_I32, _P, _P]),

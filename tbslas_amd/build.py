@@ -60,6 +60,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
                       os.path.join(BUILD, f"tbx_adjoint_q{q}.o"), [f"-DTBX_Q={q}"]))
     units.append((os.path.join(CSRC, "tbx_common.hip"), os.path.join(BUILD, "tbx_common.o"), []))
     units.append((os.path.join(CSRC, "tbx_grid.hip"), os.path.join(BUILD, "tbx_grid.o"), []))
+    units.append((os.path.join(CSRC, "tbx_batch.hip"), os.path.join(BUILD, "tbx_batch.o"), []))
     units.append((os.path.join(CSRC, "tbx_host.cpp"), os.path.join(BUILD, "tbx_host.o"),
                   ["-x", "hip"]))
 

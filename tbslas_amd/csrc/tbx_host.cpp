@@ -1223,6 +1223,246 @@ int tbx_grid_semilag_insitu(const tbx_grid *velc, tbx_tfac_fn g, void *gctx, tbx
   return TBX_OK;
 }
 
+struct tbx_batch_s {
+  int nprob = 0, bc = 0, cfg = -1, parity = 0, grid = 0;
+  std::vector<const tbx_tree *> vel, conc;
+  std::vector<double> dt;
+  std::vector<BatchProblem> host;  // table for the current configuration
+  float *d_vel32 = nullptr, *d_conc32[2] = {nullptr, nullptr};
+  std::map<int, double *> d_M;     // refit operator per degree
+  BatchProblem *d_table = nullptr;
+  unsigned long long *d_counter = nullptr;
+  int64_t total_items = 0;
+  std::vector<int64_t> conc_off;   // first FP32 coefficient of each problem
+  hipStream_t cap = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+void free_graphs(tbx_batch *b) {
+  for (int k = 0; k < 2; ++k) {
+    if (b->exec[k]) hipGraphExecDestroy(b->exec[k]);
+    if (b->graph[k]) hipGraphDestroy(b->graph[k]);
+    b->exec[k] = nullptr;
+    b->graph[k] = nullptr;
+  }
+}
+
+void free_batch(tbx_batch *b) {
+  if (!b) return;
+  hipDeviceSynchronize();
+  free_graphs(b);
+  if (b->cap) hipStreamDestroy(b->cap);
+  hipFree(b->d_vel32);
+  hipFree(b->d_conc32[0]);
+  hipFree(b->d_conc32[1]);
+  for (auto &kv : b->d_M) hipFree(kv.second);
+  hipFree(b->d_table);
+  hipFree(b->d_counter);
+  delete b;
+}
+
+// Problem table and persistent grid of configuration cfg; two graphs (one
+// per parity) of counter reset + persistent kernel.
+int batch_configure(tbx_batch *b, int cfg) {
+  if (cfg < 0 || cfg >= kBatchConfigs) return fail(TBX_EINVAL, "batch: configuration %d", cfg);
+  HIP_TRY(hipDeviceSynchronize());
+  free_graphs(b);
+  const int lpi = kBatchLpi[cfg];
+  int64_t item = 0;
+  for (int p = 0; p < b->nprob; ++p) {
+    BatchProblem &pb = b->host[p];
+    pb.item0 = item;
+    item += (pb.L + lpi - 1) / lpi;
+  }
+  b->total_items = item;
+  HIP_TRY(hipMemcpy(b->d_table, b->host.data(), sizeof(BatchProblem) * b->nprob,
+                    hipMemcpyHostToDevice));
+  int dev = 0, ncu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  b->grid = ncu * batch_blocks_per_cu(cfg);
+  for (int k = 0; k < 2; ++k) {
+    HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
+    hipError_t e = launch_batch(cfg, b->d_table, b->nprob, k, b->total_items, b->d_counter,
+                                b->grid, b->cap);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->cap, &g);
+    if (e != hipSuccess) return fail(TBX_EHIP, "batch capture: %s", hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(TBX_EHIP, "batch capture: %s", hipGetErrorString(e2));
+    b->graph[k] = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec[k], g, nullptr, nullptr, 0));
+  }
+  b->cfg = cfg;
+  return TBX_OK;
+}
+
+}  // namespace
+
+int tbx_batch_create(int nprob, const tbx_tree *const *vel, const tbx_tree *const *conc,
+                     const double *dt, int bc, tbx_batch **out) {
+  if (!out || nprob < 1 || !vel || !conc || !dt)
+    return fail(TBX_EINVAL, "tbx_batch_create: bad argument");
+  *out = nullptr;
+  int rc = check_bc(bc);
+  if (rc) return rc;
+  for (int p = 0; p < nprob; ++p) {
+    if (!vel[p] || !conc[p]) return fail(TBX_EINVAL, "tbx_batch_create: problem %d: null tree", p);
+    if (vel[p]->dof != 3 || conc[p]->dof != 1)
+      return fail(TBX_EINVAL, "tbx_batch_create: problem %d: dof %d/%d (want 3/1)", p,
+                  vel[p]->dof, conc[p]->dof);
+    const int q = conc[p]->q;
+    if (vel[p]->q != q) return fail(TBX_EUNSUPPORTED, "tbx_batch_create: problem %d: degrees", p);
+    if (q != 6 && q != 8 && q != 10)
+      return fail(TBX_EUNSUPPORTED, "tbx_batch_create: problem %d: degree %d (6, 8 or 10)", p, q);
+  }
+  tbx_batch *b = new tbx_batch_s();
+  b->nprob = nprob;
+  b->bc = bc;
+  b->vel.assign(vel, vel + nprob);
+  b->conc.assign(conc, conc + nprob);
+  b->dt.assign(dt, dt + nprob);
+  auto body = [&]() -> int {
+    int64_t nv = 0, nc = 0;
+    b->conc_off.resize(nprob);
+    for (int p = 0; p < nprob; ++p) {
+      nv += vel[p]->ncoeff_total();
+      b->conc_off[p] = nc;
+      nc += conc[p]->ncoeff_total();
+    }
+    HIP_TRY(hipMalloc(&b->d_vel32, sizeof(float) * nv + kCoeffPad));
+    HIP_TRY(hipMalloc(&b->d_conc32[0], sizeof(float) * nc + kCoeffPad));
+    HIP_TRY(hipMalloc(&b->d_conc32[1], sizeof(float) * nc + kCoeffPad));
+    HIP_TRY(hipMalloc(&b->d_table, sizeof(BatchProblem) * nprob));
+    HIP_TRY(hipMalloc(&b->d_counter, sizeof(unsigned long long)));
+    HIP_TRY(hipStreamCreate(&b->cap));
+    for (int q : {6, 8, 10}) {
+      const std::vector<double> &M = pt2coeff_cached(q);
+      double *d = nullptr;
+      HIP_TRY(hipMalloc(&d, sizeof(double) * M.size()));
+      b->d_M[q] = d;
+      HIP_TRY(hipMemcpy(d, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice));
+    }
+    b->host.resize(nprob);
+    int64_t ov = 0;
+    for (int p = 0; p < nprob; ++p) {
+      const tbx_tree *tv = vel[p], *tc = conc[p];
+      HIP_TRY(launch_to_f32(tv->d_coeff, b->d_vel32 + ov, tv->ncoeff_total(), nullptr));
+      HIP_TRY(launch_to_f32(tc->d_coeff, b->d_conc32[0] + b->conc_off[p], tc->ncoeff_total(),
+                            nullptr));
+      BatchProblem &pb = b->host[p];
+      pb = BatchProblem{};
+      pb.vel = tv->view();
+      pb.vel.coeff32 = b->d_vel32 + ov;
+      for (int k = 0; k < 2; ++k) {
+        pb.conc[k] = tc->view();
+        pb.conc[k].coeff32 = b->d_conc32[k] + b->conc_off[p];
+      }
+      pb.out32[0] = b->d_conc32[1] + b->conc_off[p];
+      pb.out32[1] = b->d_conc32[0] + b->conc_off[p];
+      pb.M = b->d_M[tc->q];
+      pb.L = tc->L;
+      pb.q = tc->q;
+      pb.bc = bc;
+      // one RK2 substep from t = dt back to 0 (semilag.inc:34-35,
+      // traj.inc:55); the velocity is stationary
+      const double tinit = 1 * dt[p], tfinal = tinit - dt[p];
+      pb.tau = (tfinal - tinit) / 1;
+      pb.htau = 0.5 * pb.tau;
+      new_nodes_host(tc->q, pb.node);
+      ov += tv->ncoeff_total();
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    return batch_configure(b, 2);
+  };
+  rc = body();
+  if (rc) {
+    free_batch(b);
+    return rc;
+  }
+  *out = b;
+  return TBX_OK;
+}
+
+int tbx_batch_destroy(tbx_batch *b) {
+  if (!b) return fail(TBX_EINVAL, "tbx_batch_destroy: null batch");
+  free_batch(b);
+  return TBX_OK;
+}
+
+int tbx_batch_set_config(tbx_batch *b, int cfg) {
+  if (!b) return fail(TBX_EINVAL, "tbx_batch_set_config: null batch");
+  return batch_configure(b, cfg);
+}
+
+int tbx_batch_get_config(const tbx_batch *b, int *cfg) {
+  if (!b || !cfg) return fail(TBX_EINVAL, "tbx_batch_get_config: null argument");
+  *cfg = b->cfg;
+  return TBX_OK;
+}
+
+int tbx_batch_tune(tbx_batch *b, double *ms, tbx_stream s) {
+  if (!b) return fail(TBX_EINVAL, "tbx_batch_tune: null batch");
+  hipStream_t st = (hipStream_t)s;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int best = -1;
+  float best_ms = 0.f;
+  int rc = TBX_OK;
+  for (int cfg = 0; cfg < kBatchConfigs && !rc; ++cfg) {
+    if ((rc = batch_configure(b, cfg))) break;
+    // a step of the current parity reads one buffer and writes the other:
+    // repeating it leaves the state unchanged
+    hipError_t e = launch_batch(cfg, b->d_table, b->nprob, b->parity, b->total_items,
+                                b->d_counter, b->grid, st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    if (e == hipSuccess)
+      e = launch_batch(cfg, b->d_table, b->nprob, b->parity, b->total_items, b->d_counter,
+                       b->grid, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e != hipSuccess) {
+      rc = fail(TBX_EHIP, "tbx_batch_tune: %s", hipGetErrorString(e));
+      break;
+    }
+    if (ms) ms[cfg] = t;
+    if (best < 0 || t < best_ms) {
+      best = cfg;
+      best_ms = t;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  return batch_configure(b, best);
+}
+
+int tbx_batch_step(tbx_batch *b, int nsteps, tbx_stream s) {
+  if (!b || nsteps < 0) return fail(TBX_EINVAL, "tbx_batch_step: bad argument");
+  for (int k = 0; k < nsteps; ++k) {
+    HIP_TRY(hipGraphLaunch(b->exec[b->parity], (hipStream_t)s));
+    b->parity ^= 1;
+  }
+  return TBX_OK;
+}
+
+int tbx_batch_get_coeffs(const tbx_batch *b, int p, double *coeff, tbx_stream s) {
+  if (!b || !coeff || p < 0 || p >= b->nprob)
+    return fail(TBX_EINVAL, "tbx_batch_get_coeffs: bad argument");
+  const size_t n = (size_t)b->conc[p]->ncoeff_total();
+  std::vector<float> tmp(n);
+  HIP_TRY(hipMemcpyAsync(tmp.data(), b->d_conc32[b->parity] + b->conc_off[p], sizeof(float) * n,
+                         hipMemcpyDeviceToHost, (hipStream_t)s));
+  HIP_TRY(hipStreamSynchronize((hipStream_t)s));
+  for (size_t i = 0; i < n; ++i) coeff[i] = tmp[i];
+  return TBX_OK;
+}
+
 int tbx_set_option(int option, int value) {
   if (option == TBX_OPT_STEP_SHAPE) {
     if (value < 0 || value >= kStepShapes)

@@ -145,6 +145,29 @@ struct GridStepArgs {
   double node[16];  // a / (N - 1)
 };
 
+// One problem of a persistent batch (tbx_batch.hip).
+struct BatchProblem {
+  TreeView vel;         // coeff32: the batch's FP32 velocity coefficients
+  TreeView conc[2];     // coeff32: the FP32 state read by a step of parity k
+  float *out32[2];      // written by a step of parity k (= conc[1 - k].coeff32)
+  const double *M;      // [P][Nc] refit operator
+  int64_t item0;        // first work item
+  int64_t L;            // concentration leaves
+  int32_t q;
+  int32_t bc;
+  double tau, htau;
+  double node[17];
+};
+// Launch configurations the batch autotuner chooses from:
+// {threads per workgroup, leaves per work item}.
+constexpr int kBatchConfigs = 6;
+constexpr int kBatchBlock[kBatchConfigs] = {256, 256, 512, 512, 512, 1024};
+constexpr int kBatchLpi[kBatchConfigs] = {1, 2, 1, 2, 4, 1};
+int batch_blocks_per_cu(int cfg);
+hipError_t launch_batch(int cfg, const BatchProblem *pb, int nprob, int parity,
+                        int64_t total_items, unsigned long long *counter, int grid,
+                        hipStream_t s);
+
 // Grid-leaf launchers (tbx_grid.hip).
 hipError_t launch_grid_eval(int mode, const TreeView &t, const double *s, const float *s32,
                             int n, int dof, double *pts, int64_t npts, int bc, int write_back,

@@ -57,3 +57,64 @@ def test_64_tracers_share_one_trajectory_set(torch, tb, orc):
     orc.semilag_insitu(ov, oc, orc.pt2coeff(q), 1, dt, 1, 0)
     got = conc.get_coeffs()
     assert np.abs(got - oc.coeff).max() <= 1e-12 * np.abs(oc.coeff).max()
+
+
+def _small_problems(tb, n=12, seed=33):
+    # a reduced C3b: depths 2..3, degrees 6/8/10, rigid rotations about
+    # random axes through the centre, a Gaussian blob each
+    from tbslas_amd import fields
+
+    u = fields.splitmix64(seed, 4 * n).reshape(n, 4)
+    probs = []
+    for p in range(n):
+        depth_u = 2 + p % 2
+        q = (6, 8, 10)[p % 3]
+        z = 2 * u[p, 0] - 1
+        phi = 2 * math.pi * u[p, 1]
+        axis = (math.sqrt(1 - z * z) * math.cos(phi), math.sqrt(1 - z * z) * math.sin(phi), z)
+        anchor, depth = tb.uniform_leaves(depth_u)
+        vcf = tb.fit_function(lambda x, a=axis: fields.rotation_axis(x, a, 2 * math.pi), anchor,
+                              depth, q, 3)
+        ccf = tb.fit_function(lambda x: fields.gaussian(x, (0.6, 0.5, 0.5), 0.12), anchor, depth,
+                              q, 1)
+        probs.append((anchor, depth, q, vcf, ccf, 0.005 * (1 + p % 3)))
+    return probs
+
+
+def test_batch_matches_per_problem_steps(torch, tb):
+    from tbslas_amd.batch import CONFIGS, Batch
+
+    probs = _small_problems(tb)
+    vel = [tb.Tree(a, d, q, 3, v) for a, d, q, v, c, dt in probs]
+    conc = [tb.Tree(a, d, q, 1, c) for a, d, q, v, c, dt in probs]
+    dts = [dt for *_, dt in probs]
+    # per-problem references in the same arithmetic (FP32 contraction,
+    # FP64 refit), three steps (timestep 1: a stationary velocity makes
+    # every step the same map).  Against FP64 the difference would be the
+    # face flips of FP32 departure points on these coarse problems (leaf
+    # faces the rotation axis crosses, fit jumps ~1e-4), not rounding.
+    refs = []
+    for (a, d, q, v, c, dt), tv in zip(probs, vel):
+        t = tb.Tree(a, d, q, 1, c)
+        for _ in range(3):
+            tb.semilag_insitu(tv, t, 1, dt, 1, 0, "fp32")
+        refs.append(t.get_coeffs())
+    results = []
+    for cfg in range(len(CONFIGS)):
+        b = Batch(vel, conc, dts, 0)
+        b.set_config(cfg)
+        b.step(3)  # graph replays
+        results.append([b.coeffs(p) for p in range(len(probs))])
+        b.close()
+    for cfg in range(1, len(CONFIGS)):  # configurations change the work split only
+        for p in range(len(probs)):
+            assert np.array_equal(results[cfg][p], results[0][p]), (cfg, p)
+    for p in range(len(probs)):  # the batch keeps its state in FP32
+        err = np.abs(results[0][p] - refs[p]).max() / np.abs(refs[p]).max()
+        assert err <= 1e-6, (p, err)
+    b = Batch(vel, conc, dts, 0)
+    ms = b.tune()
+    assert (ms > 0).all() and b.config == int(np.argmin(ms))
+    b.step(3)
+    for p in range(len(probs)):
+        assert np.array_equal(b.coeffs(p), results[0][p])
