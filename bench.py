@@ -234,6 +234,73 @@ def c3a_measure(steps: int = 2):
     return out
 
 
+def c3b_problems(n: int = 256, seed: int = 33):
+    """The batch workload: n problems with depth uniform in {3, 4, 5, 6},
+    degree in {6, 8, 10}, a rigid rotation (omega = 2 pi) about a uniformly
+    random axis through the centre, a Gaussian blob (sigma 0.08 at (0.6,
+    0.5, 0.5)) and dt = 1/200, drawn from SplitMix64(seed) as (z, phi,
+    depth, degree) per problem."""
+    from tbslas_amd import fields
+
+    u = fields.splitmix64(seed, 4 * n).reshape(n, 4)
+    out = []
+    for p in range(n):
+        z, phi = 2 * u[p, 0] - 1, 2 * math.pi * u[p, 1]
+        r = math.sqrt(max(0.0, 1 - z * z))
+        axis = (r * math.cos(phi), r * math.sin(phi), z)
+        out.append((3 + int(4 * u[p, 2]), (6, 8, 10)[int(3 * u[p, 3])], axis))
+    return out
+
+
+def c3b_measure(steps: int = 2):
+    """256 small problems (c3b_problems) in one persistent graph-captured
+    launch, FP32 contraction; the launch configuration autotuned."""
+    import torch
+
+    import tbslas_amd as tb
+    from tbslas_amd import fields
+    from tbslas_amd.batch import CONFIGS, Batch
+
+    t0 = time.perf_counter()
+    vel, conc, dts = [], [], []
+    npts = 0
+    for depth_u, q, axis in c3b_problems():
+        anchor, depth = tb.uniform_leaves(depth_u)
+        vel.append(tb.Tree(anchor, depth, q, 3,
+                           fields.rotation_coefficients(anchor, depth, q, axis, 2 * math.pi)))
+        c = tb.Tree(anchor, depth, q, 1)
+        gp = c.grid_points()
+        v = torch.exp(-((gp - torch.tensor([0.6, 0.5, 0.5], dtype=torch.float64,
+                                           device="cuda")) ** 2).sum(1) / (2 * 0.08 ** 2))
+        c.set_grid_values(v)
+        del gp, v
+        conc.append(c)
+        dts.append(1.0 / 200)
+        npts += len(depth) * (q + 1) ** 3
+    torch.cuda.empty_cache()
+    b = Batch(vel, conc, dts, 0)
+    setup = time.perf_counter() - t0
+    ms = b.tune()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    b.step(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    step_ms = e0.elapsed_time(e1) / steps
+    out = {"problems": len(vel), "departure_points_per_step": npts,
+           "interps_per_step": 3 * npts, "setup_s": setup,
+           "tuned_config": {"threads": CONFIGS[b.config][0],
+                            "leaves_per_item": CONFIGS[b.config][1]},
+           "tune_ms": {f"{t}x{lpi}": float(m) for (t, lpi), m in zip(CONFIGS, ms)},
+           "ms_per_step": step_ms, "interps_per_s": 3 * npts / (step_ms * 1e-3),
+           "departure_points_per_s": npts / (step_ms * 1e-3), "dtype": "f32"}
+    b.close()
+    del vel, conc
+    torch.cuda.empty_cache()
+    return out
+
+
 def traffic_from_profiles(kernel: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
     summary (profiles/step_traffic.txt: "kernel bytes" lines), if measured."""
@@ -263,6 +330,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="skip the tricubic-variant measurement")
     ap.add_argument("--no-c3a", action="store_true", help="skip the 64-tracer measurement")
+    ap.add_argument("--c3b", action="store_true",
+                    help="also measure the 256-problem persistent batch (about 200 GB of "
+                         "device memory, a few minutes)")
+    ap.add_argument("--c3b-only", action="store_true", help="only the 256-problem batch")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
@@ -273,6 +344,9 @@ def main():
 
     import tbslas_amd as tb
 
+    if args.c3b_only:
+        print(json.dumps({"c3b_batch": c3b_measure(2)}))
+        return
     t_setup = time.perf_counter()
     anchor, depth, ccf, vcf = build_workload()
     L = len(depth)
@@ -394,6 +468,8 @@ def main():
     if not args.no_c3a:
         torch.cuda.empty_cache()
         out["c3a_64_tracers"] = c3a_measure(2)
+    if args.c3b:
+        out["c3b_batch"] = c3b_measure(2)
     if not args.no_cpu:
         v, npts, nl, t = cpu_baseline(anchor, depth, ccf, vcf, args.cpu_leaves)
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -187,3 +187,27 @@ def gaussian_blobs(pts, centres, sigma):
     d2 = ((pts[:, None, :] - centres[None, :, :]) ** 2).sum(-1)
     return (-d2 / (2 * sigma[None, :] ** 2)).exp() if hasattr(d2, "exp") else np.exp(
         -d2 / (2 * sigma[None, :] ** 2))
+
+
+def rotation_coefficients(anchor, depth, q: int, axis, omega: float,
+                          centre=(0.5, 0.5, 0.5)) -> np.ndarray:
+    """Chebyshev coefficients [L, 3, Nc] of the rigid rotation v = omega a x
+    (p - c) on every leaf, in closed form: v is linear, so on a leaf with
+    corner k and edge h, with p = k + h (xi + 1) / 2, only T_0 (the value at
+    the leaf centre) and the three T_1 terms (omega a x e_d h / 2 for axis
+    d) are nonzero.  Packed order i(z) -> j(y) -> k(x): T_1(x) at index 1,
+    T_1(y) at index q + 1, T_1(z) at index (q + 1)(q + 2) / 2."""
+    anchor = np.asarray(anchor, dtype=np.float64).reshape(-1, 3)
+    h = 0.5 ** np.asarray(depth, dtype=np.float64)
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    d = q + 1
+    nc = d * (d + 1) * (d + 2) // 6
+    out = np.zeros((len(h), 3, nc))
+    mid = anchor * h[:, None] + 0.5 * h[:, None] - np.asarray(centre)
+    out[:, :, 0] = omega * np.cross(a, mid)
+    for ax, idx in ((0, 1), (1, d), (2, d * (d + 1) // 2)):
+        e = np.zeros(3)
+        e[ax] = 1.0
+        out[:, :, idx] = omega * np.cross(a, e)[None, :] * (0.5 * h)[:, None]
+    return out

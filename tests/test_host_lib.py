@@ -156,3 +156,16 @@ def test_random_poly_velocity_is_bounded_and_polynomial():
     cf = tb.fit_function(f, anchor, depth, 4, 3)
     cf2 = tb.fit_function(f, anchor, depth, 6, 3)
     assert np.abs(cf).max() > 0 and np.isfinite(cf2).all()
+
+
+def test_rotation_coefficients_closed_form():
+    # the closed-form coefficients of a rigid rotation equal the fitted ones
+    import tbslas_amd as tb
+    from tbslas_amd import fields
+
+    anchor, depth = tb.uniform_leaves(2)
+    axis = (0.3, -0.5, 0.81)
+    for q in (1, 6):
+        fit = tb.fit_function(lambda p: fields.rotation_axis(p, axis, 2.5), anchor, depth, q, 3)
+        cf = fields.rotation_coefficients(anchor, depth, q, axis, 2.5)
+        assert np.abs(fit - cf).max() <= 1e-13
