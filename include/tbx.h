@@ -285,9 +285,13 @@ int tbx_semilag_insitu_vjp(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
  * loss = 0.5 |c_T - c_target|^2 (returned on the host), and its gradients:
  * grad_c0 [L][dof][Nc] and grad_vel [L_vel][3][Nc_vel] (may be NULL),
  * overwritten.  States are checkpointed every checkpoint_every steps and
- * recomputed segment by segment in the backward sweep; a stationary
- * velocity (g NULL) scatters its cotangent once.  conc holds c_T on
- * return. */
+ * recomputed segment by segment in the backward sweep; the departure
+ * points of each recomputed step (of step 1 alone for a stationary
+ * velocity, g NULL, whose trajectories are the same every step) are kept
+ * on the device, 24 bytes per grid point and step, while they fit in half
+ * the free device memory, and the adjoint steps read them instead of
+ * integrating again.  A stationary velocity scatters its cotangent once.
+ * conc holds c_T on return. */
 int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
                     tbx_tree *conc, int nsteps, double dt, int bc,
                     uint32_t flags, const double *c_target, int checkpoint_every,

@@ -66,88 +66,136 @@ struct AdjBasis {
   }
 };
 
-// sum_m c_m bx_k by_j bz_i over the packed triangle, c wave-uniform.
-__device__ __forceinline__ double contract3(const double *c, const double *bx,
-                                            const double *by, const double *bz) {
-  double u = 0.0;
+// Gradient (d/dx, d/dy, d/dz) of one component at the point of b, c the
+// component's coefficients in b's leaf.  One pass over the packed triangle:
+// every coefficient feeds the x-derivative and the value sums along k, and
+// the (i, j) partial sums are then combined with the y and z bases and
+// derivatives.
+__device__ __forceinline__ void grad3(const double *c, const AdjBasis &b, double &gx,
+                                      double &gy, double &gz) {
+  double ux = 0.0, uy = 0.0, uz = 0.0;
   int idx = 0;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    double v = 0.0;
+    double vx = 0.0, vy = 0.0, v = 0.0;
 #pragma unroll
     for (int j = 0; i + j < D; ++j) {
-      double w = 0.0;
+      double wx = 0.0, w = 0.0;
 #pragma unroll
-      for (int k = 0; i + j + k < D; ++k) w = __builtin_fma(bx[k], load_uniform(c + idx++), w);
-      v = __builtin_fma(by[j], w, v);
+      for (int k = 0; i + j + k < D; ++k) {
+        const double cm = load_uniform(c + idx++);
+        wx = __builtin_fma(b.dx[k], cm, wx);
+        w = __builtin_fma(b.tx[k], cm, w);
+      }
+      vx = __builtin_fma(b.ty[j], wx, vx);
+      vy = __builtin_fma(b.dy[j], w, vy);
+      v = __builtin_fma(b.ty[j], w, v);
     }
-    u = __builtin_fma(bz[i], v, u);
+    ux = __builtin_fma(b.tz[i], vx, ux);
+    uy = __builtin_fma(b.tz[i], vy, uy);
+    uz = __builtin_fma(b.dz[i], v, uz);
   }
-  return u;
+  gx = ux * b.s2;
+  gy = uy * b.s2;
+  gz = uz * b.s2;
 }
 
-// Gradient (d/dx, d/dy, d/dz) of one component at the point of b, c the
-// component's coefficients in b's leaf.
-__device__ __forceinline__ void grad3(const double *c, const AdjBasis &b, double &gx,
-                                      double &gy, double &gz) {
-  gx = contract3(c, b.dx, b.ty, b.tz) * b.s2;
-  gy = contract3(c, b.tx, b.dy, b.tz) * b.s2;
-  gz = contract3(c, b.tx, b.ty, b.dz) * b.s2;
+// Cross-lane exchanges of one double.  swap_add32 / swap_add16: with lo
+// and hi two blocks of values, lanes whose bit 5 (bit 4) is clear return
+// lo + lo of lane l ^ 32 (l ^ 16) and the others hi + hi of that lane
+// (v_permlane32_swap / v_permlane16_swap exchange the upper lanes of the
+// first operand with the lower lanes of the second).  dpp_pair: the value
+// of the partner lane of a DPP pattern (row_ror:8 = l ^ 8 in a row,
+// row_half_mirror = 7 - l in each eight, quad_perm = l ^ 2, l ^ 1).
+__device__ __forceinline__ double swap_add32(double lo, double hi) {
+  const auto a = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(lo),
+                                                  (unsigned)__double2loint(hi), false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(lo),
+                                                  (unsigned)__double2hiint(hi), false, false);
+  return __hiloint2double((int)b[0], (int)a[0]) + __hiloint2double((int)b[1], (int)a[1]);
 }
 
-// Sum of v over the 64 lanes, in lane 63 (all lanes must be active):
-// row prefix sums by DPP row shifts, then the row totals combined by the
-// row broadcasts.  Lanes whose source is outside the row read 0.
-__device__ __forceinline__ double dpp_shift(double v, int ctrl_id) {
-  const int lo = __double2loint(v), hi = __double2hiint(v);
-  int rlo, rhi;
-  switch (ctrl_id) {
-    case 0:
-      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x111, 0xf, 0xf, true);
-      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x111, 0xf, 0xf, true);
-      break;
-    case 1:
-      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x112, 0xf, 0xf, true);
-      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x112, 0xf, 0xf, true);
-      break;
-    case 2:
-      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x114, 0xf, 0xf, true);
-      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x114, 0xf, 0xf, true);
-      break;
-    case 3:
-      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x118, 0xf, 0xf, true);
-      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x118, 0xf, 0xf, true);
-      break;
-    case 4:  // row_bcast:15 into rows 1 and 3
-      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x142, 0xa, 0xf, false);
-      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x142, 0xa, 0xf, false);
-      break;
-    default:  // row_bcast:31 into rows 2 and 3
-      rlo = __builtin_amdgcn_update_dpp(0, lo, 0x143, 0xc, 0xf, false);
-      rhi = __builtin_amdgcn_update_dpp(0, hi, 0x143, 0xc, 0xf, false);
-      break;
+__device__ __forceinline__ double swap_add16(double lo, double hi) {
+  const auto a = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(lo),
+                                                  (unsigned)__double2loint(hi), false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(lo),
+                                                  (unsigned)__double2hiint(hi), false, false);
+  return __hiloint2double((int)b[0], (int)a[0]) + __hiloint2double((int)b[1], (int)a[1]);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_pair(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// One halving step over b[0, 2H): the lane keeps the upper half when
+// `upper`, sends the other half to its partner and adds what it receives.
+template <int CTRL, int H, int N>
+__device__ __forceinline__ void halve(double (&b)[N], bool upper) {
+#pragma unroll
+  for (int t = 0; t < H; ++t) {
+    const double send = upper ? b[t] : b[t + H];
+    const double keep = upper ? b[t + H] : b[t];
+    b[t] = keep + dpp_pair<CTRL>(send);
   }
-  return __hiloint2double(rhi, rlo);
 }
 
-__device__ __forceinline__ double wave_sum63(double v) {
-  v = v + dpp_shift(v, 0);
-  v = v + dpp_shift(v, 1);
-  v = v + dpp_shift(v, 2);
-  v = v + dpp_shift(v, 3);
-  v = v + dpp_shift(v, 4);
-  v = v + dpp_shift(v, 5);
-  return v;
+// Reduce-scatter over the wave (all 64 lanes active): on return lane l
+// holds the sum over all lanes of b[l] (N = 64), or, for N = 32, lanes l
+// and l ^ 1 both hold the sum of b[l >> 1].  Recursive halving: at every
+// step the two partners hold the same block of indices and each keeps one
+// half, so N - 1 exchanges replace N full reductions.
+template <int N>
+__device__ __forceinline__ double wave_reduce_scatter(double (&b)[N]) {
+  static_assert(N == 64 || N == 32, "chunk of 32 or 64");
+  const unsigned l = threadIdx.x & 63u;
+  constexpr int H1 = N / 2, H2 = N / 4, H3 = N / 8, H4 = N / 16, H5 = N / 32;
+#pragma unroll
+  for (int t = 0; t < H1; ++t) b[t] = swap_add32(b[t], b[t + H1]);
+#pragma unroll
+  for (int t = 0; t < H2; ++t) b[t] = swap_add16(b[t], b[t + H2]);
+  halve<0x128, H3>(b, (l & 8u) != 0);
+  halve<0x141, H4>(b, (l & 4u) != 0);
+  halve<0x4e, H5>(b, (l & 2u) != 0);
+  if constexpr (N == 64)
+    halve<0xb1, 1>(b, (l & 1u) != 0);
+  else
+    b[0] = b[0] + dpp_pair<0xb1>(b[0]);
+  return b[0];
 }
+
+// (i, j, k) of packed triangle index m (order i(z) -> j(y) -> k(x)).
+struct Tri {
+  int i[NC], j[NC], k[NC];
+  constexpr Tri() : i(), j(), k() {
+    int m = 0;
+    for (int a = 0; a < D; ++a)
+      for (int b = 0; a + b < D; ++b)
+        for (int c = 0; a + b + c < D; ++c) {
+          i[m] = a;
+          j[m] = b;
+          k[m] = c;
+          ++m;
+        }
+  }
+};
+constexpr Tri kTri{};
 
 // Transposed interpolation of ncomp weighted components: for every leaf
 // held by a lane of the wave, G[leaf][c][m] += sum over its lanes of
 // w(c) B_m(point).  All lanes of the wave must be active; each lane's basis
-// b is in its own leaf.  One atomic add per coefficient and leaf per wave.
-template <typename W>
+// b is in its own leaf.  The coefficients go in chunks of CH (64 or 32:
+// CH doubles of registers per lane) through the wave reduce-scatter, one
+// coalesced atomic add per lane and chunk.
+template <int CH, typename W>
 __device__ __forceinline__ void scatter_transposed(double *G, int ncomp, const AdjBasis &b,
                                                    W weight) {
-  const bool last = (threadIdx.x & 63) == 63;
+  constexpr int PER = 64 / CH;  // lanes per reduced coefficient
+  const int lane = (int)(threadIdx.x & 63u);
+  const bool writer = (lane % PER) == 0;
+  const int slot = lane / PER;
   uint64_t pending = __builtin_amdgcn_ballot_w64(true);
   while (pending) {
     const int lf = __builtin_amdgcn_readlane(b.leaf, __builtin_ctzll(pending));
@@ -157,23 +205,26 @@ __device__ __forceinline__ void scatter_transposed(double *G, int ncomp, const A
     for (int c = 0; c < ncomp; ++c) {
       const double wc = mine ? weight(c) : 0.0;
       double *g = G + ((size_t)lf * ncomp + c) * NC;
-      int idx = 0;
 #pragma unroll
-      for (int i = 0; i < D; ++i) {
+      for (int m0 = 0; m0 < NC; m0 += CH) {
+        double buf[CH];
 #pragma unroll
-        for (int j = 0; i + j < D; ++j) {
-          const double tij = wc * b.tz[i] * b.ty[j];
-#pragma unroll
-          for (int k = 0; i + j + k < D; ++k) {
-            const double s = wave_sum63(tij * b.tx[k]);
-            if (last) atomicAdd(g + idx, s);
-            ++idx;
-          }
+        for (int r = 0; r < CH; ++r) {
+          const int m = m0 + r;
+          buf[r] = m < NC ? wc * b.tz[kTri.i[m < NC ? m : 0]] * b.ty[kTri.j[m < NC ? m : 0]] *
+                                b.tx[kTri.k[m < NC ? m : 0]]
+                          : 0.0;
         }
+        const double s = wave_reduce_scatter(buf);
+        if (writer && m0 + slot < NC) atomicAdd(g + m0 + slot, s);
       }
     }
   }
 }
+
+// Chunk of the concentration kernel's scatter: 64 while the basis arrays
+// leave room for it, 32 above degree 10.
+constexpr int kConcChunk = D <= 11 ? 64 : 32;
 
 // Arrival point of global grid index gi (tree_utils.h:480-491).
 __device__ __forceinline__ void arrival_point(const TreeView &conc, const StepArgs &a,
@@ -189,9 +240,13 @@ __device__ __forceinline__ void arrival_point(const TreeView &conc, const StepAr
 
 // Cotangent of c and, when A is given, the per-point velocity cotangent
 // factor a_p += sum_c u_{p,c} grad c_c(D_p).  ua[al][c][am] = (R^T lam')
-// at arrival point (al, am).  MODE is the forward's trajectory arithmetic.
-template <int MODE, int BLOCK>
+// at arrival point (al, am).  MODE is the forward's trajectory arithmetic;
+// with STORED the departure points are read from dep[3 gi] (the forward
+// step's own output, wrapped as the concentration saw them) instead of
+// being integrated again.
+template <int MODE, int BLOCK, bool STORED>
 __global__ __launch_bounds__(BLOCK) void k_adj_conc(VelSource vel, TreeView conc, StepArgs a,
+                                                    const double *__restrict__ dep,
                                                     const double *__restrict__ ua,
                                                     double *__restrict__ lam,
                                                     double *__restrict__ A) {
@@ -204,18 +259,24 @@ __global__ __launch_bounds__(BLOCK) void k_adj_conc(VelSource vel, TreeView conc
   double p3[3];
   {
     const int64_t i = i0 + orig < a.npts ? i0 + orig : a.npts - 1;
-    double x, y, z;
-    arrival_point(conc, a, i, x, y, z);
-    rk2_substep<D, MODE, kVelTree>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[0], a.g[1], a.ts[0],
-                                   a.ts[1], x, y, z);
-    if (a.bc == TBX_BC_PERIODIC) {
-      x = wrap1(x);
-      y = wrap1(y);
-      z = wrap1(z);
+    if constexpr (STORED) {
+      p3[0] = dep[3 * i + 0];
+      p3[1] = dep[3 * i + 1];
+      p3[2] = dep[3 * i + 2];
+    } else {
+      double x, y, z;
+      arrival_point(conc, a, i, x, y, z);
+      rk2_substep<D, MODE, kVelTree>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[0], a.g[1],
+                                     a.ts[0], a.ts[1], x, y, z);
+      if (a.bc == TBX_BC_PERIODIC) {
+        x = wrap1(x);
+        y = wrap1(y);
+        z = wrap1(z);
+      }
+      p3[0] = x;
+      p3[1] = y;
+      p3[2] = z;
     }
-    p3[0] = x;
-    p3[1] = y;
-    p3[2] = z;
   }
   int lf = locate(conc, p3[0], p3[1], p3[2]);
   regroup(sh, orig, lf, p3);
@@ -245,14 +306,18 @@ __global__ __launch_bounds__(BLOCK) void k_adj_conc(VelSource vel, TreeView conc
       A[3 * gi + 2] += az;
     }
   }
-  scatter_transposed(lam, dof, bs, weight);
+  scatter_transposed<kConcChunk>(lam, dof, bs, weight);
 }
+
+// Waves per SIMD asked of the velocity kernel: two fit without spills up
+// to degree 8 (the register budget halves to 256).
+constexpr int kVelWaves = D <= 9 ? 2 : 1;
 
 // Velocity cotangent: for D = x + tau g2 V(h), h = x + (tau/2) g1 V(x),
 //   a . dD = tau g2 a . dV(h) + tau g2 (tau/2) g1 (J_V(h)^T a) . dV(x),
 // scattered into G[leaf][k][m] at h (in h's leaf) and at x (in x's leaf).
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_adj_vel(VelSource vel, TreeView conc, StepArgs a,
+__global__ __launch_bounds__(kBlock, kVelWaves) void k_adj_vel(VelSource vel, TreeView conc, StepArgs a,
                                                     const double *__restrict__ A,
                                                     double *__restrict__ G) {
   const int64_t i0 = (int64_t)blockIdx.x * kBlock;
@@ -298,12 +363,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_vel(VelSource vel, TreeView conc
     }
   });
   const double fh = a.tau * g2, fx = a.tau * g2 * a.htau * g1;
-  scatter_transposed(G, 3, bh, [&](int c) {
+  scatter_transposed<32>(G, 3, bh, [&](int c) {
     return fh * (c == 0 ? ax : c == 1 ? ay : az);
   });
   AdjBasis bx;
   bx.build(V, x, y, z);
-  scatter_transposed(G, 3, bx, [&](int c) {
+  scatter_transposed<32>(G, 3, bx, [&](int c) {
     return fx * (c == 0 ? jx : c == 1 ? jy : jz);
   });
 }
@@ -315,15 +380,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_vel(VelSource vel, TreeView conc
 
 hipError_t TBX_CAT(launch_adj_conc_q, TBX_Q)(int mode, const VelSource &vel,
                                              const TreeView &conc, const StepArgs &a,
-                                             const double *ua, double *lam, double *A,
-                                             hipStream_t s) {
+                                             const double *dep, const double *ua, double *lam,
+                                             double *A, hipStream_t s) {
   if (a.npts <= 0) return hipSuccess;
   const dim3 g((unsigned)(8 * a.per_xcd)), blk(kAdjBlock);
-  if (mode == kFma64)
-    hipLaunchKernelGGL((k_adj_conc<kFma64, kAdjBlock>), g, blk, 0, s, vel, conc, a, ua, lam, A);
+  if (dep)  // the stored trajectories carry the forward's arithmetic
+    hipLaunchKernelGGL((k_adj_conc<kFma64, kAdjBlock, true>), g, blk, 0, s, vel, conc, a, dep,
+                       ua, lam, A);
+  else if (mode == kFma64)
+    hipLaunchKernelGGL((k_adj_conc<kFma64, kAdjBlock, false>), g, blk, 0, s, vel, conc, a, dep,
+                       ua, lam, A);
   else if (mode == kExact64)
-    hipLaunchKernelGGL((k_adj_conc<kExact64, kAdjBlock>), g, blk, 0, s, vel, conc, a, ua, lam,
-                       A);
+    hipLaunchKernelGGL((k_adj_conc<kExact64, kAdjBlock, false>), g, blk, 0, s, vel, conc, a,
+                       dep, ua, lam, A);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

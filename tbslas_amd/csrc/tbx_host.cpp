@@ -363,12 +363,12 @@ hipError_t dispatch_step(int q, int mode, const VelSource &vel,
 }
 
 hipError_t dispatch_adj_conc(int q, int mode, const VelSource &vel, const TreeView &conc,
-                             const StepArgs &a, const double *ua, double *lam, double *A,
-                             hipStream_t s) {
+                             const StepArgs &a, const double *dep, const double *ua,
+                             double *lam, double *A, hipStream_t s) {
   switch (q) {
 #define X(Q) \
   case Q:    \
-    return launch_adj_conc_q##Q(mode, vel, conc, a, ua, lam, A, s);
+    return launch_adj_conc_q##Q(mode, vel, conc, a, dep, ua, lam, A, s);
     TBX_Q_CASES(X)
 #undef X
   }
@@ -900,8 +900,10 @@ int adj_args(const AdjCtx &c, const tbx_tree *conc, int timestep, double dt, int
 
 // lam_out = E_D^T R^T lam_next for the step `timestep` from conc's current
 // coefficients; A (may be NULL) += the per-point velocity cotangent factors.
+// dep (may be NULL): the step's departure points as its forward wrote them.
 int adj_conc_step(const AdjCtx &c, tbx_tree *conc, int timestep, double dt, int bc,
-                  const double *lam_next, double *lam_out, double *A, hipStream_t st) {
+                  const double *lam_next, double *lam_out, double *A, const double *dep,
+                  hipStream_t st) {
   const int P = conc->P(), nc = conc->nc;
   const int64_t rows = conc->L * conc->dof;
   if (rows > INT32_MAX) return fail(TBX_EOVERFLOW, "adjoint: %lld rows", (long long)rows);
@@ -915,8 +917,8 @@ int adj_conc_step(const AdjCtx &c, tbx_tree *conc, int timestep, double dt, int 
   StepArgs a;
   int rc = adj_args(c, conc, timestep, dt, bc, kAdjBlock, &a);
   if (rc) return rc;
-  HIP_TRY(dispatch_adj_conc(conc->q, c.mode, c.vs, conc->view(), a, conc->d_vals, lam_out, A,
-                            st));
+  HIP_TRY(dispatch_adj_conc(conc->q, c.mode, c.vs, conc->view(), a, dep, conc->d_vals, lam_out,
+                            A, st));
   return TBX_OK;
 }
 
@@ -952,7 +954,7 @@ int tbx_semilag_insitu_vjp(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, const
   if (!lam_next || !lam_out) return fail(TBX_EINVAL, "tbx_semilag_insitu_vjp: null cotangent");
   if (grad_vel && (rc = ensure_adjA(conc, st))) return rc;
   if ((rc = adj_conc_step(c, conc, timestep, dt, bc, lam_next, lam_out,
-                          grad_vel ? conc->d_adjA : nullptr, st)))
+                          grad_vel ? conc->d_adjA : nullptr, nullptr, st)))
     return rc;
   if (grad_vel) return adj_vel_step(c, conc, timestep, dt, bc, conc->d_adjA, grad_vel, st);
   return TBX_OK;
@@ -971,11 +973,22 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
   const int K = checkpoint_every;
   const size_t nco = (size_t)conc->ncoeff_total(), bytes = sizeof(double) * nco;
   const int nck = nsteps / K + 1;
-  std::vector<double *> ck(nck, nullptr), seg(K, nullptr);
+  const bool stationary = (c.v.g == nullptr);
+  // Trajectories kept on the device: a stationary velocity has the same
+  // departure points every step (one array); otherwise every recomputed
+  // step of a segment keeps its own.  Kept only while they fit in half the
+  // free device memory; the adjoint kernel integrates them again otherwise.
+  const size_t dep_bytes = sizeof(double) * 3 * (size_t)conc->L * conc->P();
+  size_t free_mem = 0, total_mem = 0;
+  HIP_TRY(hipMemGetInfo(&free_mem, &total_mem));
+  const size_t ndep = stationary ? 1 : (size_t)(K > 1 ? K - 1 : 0);
+  const bool keep_dep = ndep > 0 && ndep * dep_bytes <= free_mem / 2;
+  std::vector<double *> ck(nck, nullptr), seg(K, nullptr), deps(keep_dep ? ndep : 0, nullptr);
   double *lam_a = nullptr, *lam_b = nullptr, *cT = nullptr;
   auto release = [&]() {
     for (double *p : ck) hipFree(p);
     for (double *p : seg) hipFree(p);
+    for (double *p : deps) hipFree(p);
     hipFree(lam_a);
     hipFree(lam_b);
     hipFree(cT);
@@ -983,14 +996,17 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
   auto body = [&]() -> int {
     for (auto &p : ck) HIP_TRY(hipMalloc(&p, bytes));
     for (auto &p : seg) HIP_TRY(hipMalloc(&p, bytes));
+    for (auto &p : deps) HIP_TRY(hipMalloc(&p, dep_bytes));
     HIP_TRY(hipMalloc(&lam_a, bytes));
     HIP_TRY(hipMalloc(&lam_b, bytes));
     HIP_TRY(hipMalloc(&cT, bytes));
     const tbx_velocity &v = c.v;
-    // forward sweep, keeping c_0, c_K, c_2K, ...
+    // forward sweep, keeping c_0, c_K, c_2K, ... (and, for a stationary
+    // velocity, the trajectories of step 1)
     HIP_TRY(hipMemcpyAsync(ck[0], conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
     for (int n = 1; n <= nsteps; ++n) {
-      int r = step_insitu(&v, conc, n, dt, 1, bc, flags, nullptr, st, "tbx_adjoint_run");
+      double *dep = (stationary && keep_dep && n == 1) ? deps[0] : nullptr;
+      int r = step_insitu(&v, conc, n, dt, 1, bc, flags, dep, st, "tbx_adjoint_run");
       if (r) return r;
       if (n % K == 0 && n / K < nck)
         HIP_TRY(hipMemcpyAsync(ck[n / K], conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
@@ -1008,7 +1024,6 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
       return fail(TBX_EHIP, "rocblas_ddot failed");
     if (loss) *loss = 0.5 * dot;
     // backward sweep by segments, recomputing c_n from the checkpoints
-    const bool stationary = (v.g == nullptr);
     if (grad_vel) {
       HIP_TRY(hipMemsetAsync(grad_vel, 0, sizeof(double) * (size_t)vel->L * 3 * vel->nc, st));
       int r = ensure_adjA(conc, st);
@@ -1020,7 +1035,8 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
       HIP_TRY(hipMemcpyAsync(conc->d_coeff, seg[0], bytes, hipMemcpyDeviceToDevice, st));
       conc->f32_stale = true;
       for (int n = s0 + 1; n < s1; ++n) {
-        int r = step_insitu(&v, conc, n, dt, 1, bc, flags, nullptr, st, "tbx_adjoint_run");
+        double *dep = (!stationary && keep_dep) ? deps[n - s0 - 1] : nullptr;
+        int r = step_insitu(&v, conc, n, dt, 1, bc, flags, dep, st, "tbx_adjoint_run");
         if (r) return r;
         HIP_TRY(hipMemcpyAsync(seg[n - s0], conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
       }
@@ -1036,7 +1052,13 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
           }
           A = conc->d_adjA;
         }
-        int r = adj_conc_step(c, conc, n, dt, bc, lam_a, lam_b, A, st);
+        // the segment's last step was not recomputed: its trajectories are
+        // integrated again unless the velocity is stationary
+        const double *dep = !keep_dep      ? nullptr
+                            : stationary   ? deps[0]
+                            : n < s1       ? deps[n - s0 - 1]
+                                           : nullptr;
+        int r = adj_conc_step(c, conc, n, dt, bc, lam_a, lam_b, A, dep, st);
         if (r) return r;
         if (grad_vel && !stationary &&
             (r = adj_vel_step(c, conc, n, dt, bc, conc->d_adjA, grad_vel, st)))

@@ -111,8 +111,8 @@ constexpr int kAdjBlock = 256;  // workgroup of the adjoint concentration kernel
                                hipStream_t s);                                \
   hipError_t launch_adj_conc_q##Q(int mode, const VelSource &vel,             \
                                   const TreeView &conc, const StepArgs &a,    \
-                                  const double *ua, double *lam, double *A,   \
-                                  hipStream_t s);                             \
+                                  const double *dep, const double *ua,        \
+                                  double *lam, double *A, hipStream_t s);     \
   hipError_t launch_adj_vel_q##Q(int mode, const VelSource &vel,              \
                                  const TreeView &conc, const StepArgs &a,     \
                                  const double *A, double *G, hipStream_t s);

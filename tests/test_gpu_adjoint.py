@@ -211,3 +211,32 @@ def test_autograd_function_matches_driver(torch, tb):
     assert abs(float(loss) - l2) <= 1e-12 * l2
     assert float((c.grad - gc).abs().max()) <= 1e-11 * float(gc.abs().max())
     assert float((v.grad - gv).abs().max()) <= 1e-11 * float(gv.abs().max())
+
+
+@pytest.mark.parametrize("bc,timed,K", [(1, True, 2), (0, True, 3), (0, False, 2)])
+def test_run_equals_chained_vjp(torch, tb, bc, timed, K):
+    # adjoint_run keeps the trajectories of the recomputed steps (of step 1
+    # for a stationary velocity) and reads them back; the chain of single
+    # vjp steps integrates every trajectory again.  Same gradients.
+    from tbslas_amd.adjoint import adjoint_run, semilag_insitu_vjp
+
+    g = (lambda t: math.cos(math.pi * t)) if timed else None
+    _, _, vel, conc, c0 = _setup(tb, 64, 5, 1)
+    dt, T = 0.03, 5
+    c0d = torch.from_numpy(c0).cuda()
+    target = 0.8 * c0d
+    states = [c0d.clone()]
+    conc.set_coeffs(c0d)
+    for n in range(1, T + 1):
+        tb.semilag_insitu(vel, conc, n, dt, 1, bc, "fp64", g=g)
+        states.append(conc.get_coeffs_device().clone())
+    lam = states[-1] - target
+    gv = torch.zeros((vel.L, 3, vel.nc), dtype=torch.float64, device="cuda")
+    for n in range(T, 0, -1):
+        conc.set_coeffs(states[n - 1])
+        lam = semilag_insitu_vjp(vel, conc, n, dt, lam, bc, "fp64", g=g, grad_vel=gv)
+    conc.set_coeffs(c0d)
+    loss, gc, gvr = adjoint_run(vel, conc, T, dt, target, bc, "fp64", g=g, checkpoint_every=K)
+    assert abs(loss - 0.5 * float(((states[-1] - target) ** 2).sum())) <= 1e-13 * loss
+    assert float((gc - lam).abs().max()) <= 1e-12 * float(lam.abs().max())
+    assert float((gvr - gv).abs().max()) <= 1e-11 * float(gv.abs().max())
