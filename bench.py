@@ -171,6 +171,61 @@ def c5_measure(steps: int):
     return out
 
 
+def c4_measure():
+    """The FP64 adjoint at full size: uniform depth 5 (32,768 leaves), q = 8,
+    stationary random Chebyshev velocity (SplitMix64 seed 4, |v| <= 1), a
+    Gaussian blob, 20 steps of dt = 0.01, FreeSpace, checkpoints every 5
+    steps.  The adjoint run = 20 forward steps + 16 recomputed + 20 adjoint
+    steps + one velocity scatter; the adjoint step is the run less its 36
+    forward steps, per step (the velocity scatter spread over the 20)."""
+    import torch
+
+    import tbslas_amd as tb
+    from tbslas_amd import fields
+    from tbslas_amd.adjoint import adjoint_run
+
+    q, d, T, dt, K = 8, 5, 20, 0.01, 5
+    anchor, depth = tb.uniform_leaves(d)
+    vc = tb.fit_function(fields.random_poly_velocity(q, seed=4), anchor, depth, q, 3)
+    c0 = tb.fit_function(lambda p: fields.gaussian(p, (0.5, 0.5, 0.5), 0.1), anchor, depth, q, 1)
+    vel, conc = tb.Tree(anchor, depth, q, 3, vc), tb.Tree(anchor, depth, q, 1, c0)
+    c0d = torch.from_numpy(c0).cuda()
+    target = 0.9 * c0d
+
+    def forward():
+        conc.set_coeffs(c0d)
+        for n in range(1, T + 1):
+            tb.semilag_insitu(vel, conc, n, dt, 1, 0, "fp64")
+
+    def run():
+        conc.set_coeffs(c0d)
+        adjoint_run(vel, conc, T, dt, target, 0, "fp64", checkpoint_every=K)
+
+    def timed(f, reps):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    fwd_ms = timed(forward, 3)
+    run_ms = timed(run, 3)
+    recomputed = T - (T + K - 1) // K
+    step_ms = fwd_ms / T
+    adj_ms = (run_ms - (T + recomputed) * step_ms) / T
+    del vel, conc
+    return {"leaves": len(depth), "q": q, "steps": T, "checkpoint_every": K,
+            "grid_points": len(depth) * (q + 1) ** 3,
+            "forward_step_ms": step_ms, "adjoint_run_ms": run_ms,
+            "plain_forward_ms": fwd_ms, "adjoint_step_ms": adj_ms,
+            "adjoint_step_over_forward_step": adj_ms / step_ms,
+            "adjoint_run_over_plain_forward": run_ms / fwd_ms}
+
+
 def c3a_measure(steps: int = 2):
     """Batched tracers at full size: 64 seeded Gaussian tracers (SplitMix64
     seed 3) on a uniform depth-6 tree (262,144 leaves), q = 8, carried by one
@@ -330,6 +385,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="skip the tricubic-variant measurement")
     ap.add_argument("--no-c3a", action="store_true", help="skip the 64-tracer measurement")
+    ap.add_argument("--no-c4", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--c3b", action="store_true",
                     help="also measure the 256-problem persistent batch (about 200 GB of "
                          "device memory, a few minutes)")
@@ -468,6 +524,9 @@ def main():
     if not args.no_c3a:
         torch.cuda.empty_cache()
         out["c3a_64_tracers"] = c3a_measure(2)
+    if not args.no_c4:
+        torch.cuda.empty_cache()
+        out["c4_adjoint"] = c4_measure()
     if args.c3b:
         out["c3b_batch"] = c3b_measure(2)
     if not args.no_cpu:
