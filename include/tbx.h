@@ -333,8 +333,13 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
  * regrouped by leaf before each evaluation, 256 / 512 threads (budget 2
  * waves per SIMD) and 1024 / 512 threads (budget 4).  6 is the default;
  * shapes other than 0 and 6 exist for q = 8 and 14 only (others run 6).
- * Every shape gives the same results. */
-enum { TBX_OPT_STEP_SHAPE = 1 };
+ * Every shape gives the same results.
+ * TBX_OPT_TRACER_MFMA (default 1): a concentration with a multiple of 16
+ * components (many tracers on one trajectory set) has its components
+ * contracted on the matrix cores in TBX_FP64 and TBX_FP32 steps (512-thread
+ * regrouped workgroups, whatever the step shape); 0 keeps the vector
+ * contraction.  The two agree to rounding (summation order). */
+enum { TBX_OPT_STEP_SHAPE = 1, TBX_OPT_TRACER_MFMA = 2 };
 int tbx_set_option(int option, int value);
 int tbx_get_option(int option, int *value);
 

@@ -22,6 +22,7 @@ WRITE_BACK_WRAP = 0x10
 MAX_Q = 16
 MAX_DEPTH = 21
 OPT_STEP_SHAPE = 1
+OPT_TRACER_MFMA = 2
 VEL_TREE = 0
 VEL_EXTRAP = 1
 VEL_SET = 2

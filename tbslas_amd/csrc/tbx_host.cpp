@@ -28,6 +28,7 @@ namespace {
 
 thread_local std::string g_err;
 std::atomic<int> g_step_shape{kDefaultStepShape};
+std::atomic<int> g_tracer_mfma{1};
 
 int fail(int code, const char *fmt, ...) {
   char buf[512];
@@ -775,6 +776,10 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
   StepArgs a{};
   const int want = g_step_shape.load();
   a.shape = vs.kind == kVelTree && step_shape_built(want, conc->q) ? want : kDefaultStepShape;
+  // many components (tracers sharing the trajectories) on the matrix cores
+  a.mfma = g_tracer_mfma.load() && vs.kind == kVelTree && (mode == kFma64 || mode == kFp32) &&
+           conc->dof >= 16 && conc->dof % 16 == 0;
+  if (a.mfma) a.shape = kMfmaStepShape;
   a.npts = conc->L * P;
   a.nblocks = (a.npts + kStepShapeBlock[a.shape] - 1) / kStepShapeBlock[a.shape];
   a.per_xcd = (a.nblocks + 7) / 8;
@@ -1493,6 +1498,12 @@ int tbx_set_option(int option, int value) {
     g_step_shape = value;
     return TBX_OK;
   }
+  if (option == TBX_OPT_TRACER_MFMA) {
+    if (value != 0 && value != 1)
+      return fail(TBX_EINVAL, "tbx_set_option: tracer MFMA switch %d is not 0 or 1", value);
+    g_tracer_mfma = value;
+    return TBX_OK;
+  }
   return fail(TBX_EINVAL, "tbx_set_option: unknown option %d", option);
 }
 
@@ -1500,6 +1511,10 @@ int tbx_get_option(int option, int *value) {
   if (!value) return fail(TBX_EINVAL, "tbx_get_option: null output");
   if (option == TBX_OPT_STEP_SHAPE) {
     *value = g_step_shape;
+    return TBX_OK;
+  }
+  if (option == TBX_OPT_TRACER_MFMA) {
+    *value = g_tracer_mfma;
     return TBX_OK;
   }
   return fail(TBX_EINVAL, "tbx_get_option: unknown option %d", option);

@@ -76,9 +76,13 @@ constexpr int kDefaultStepShape = 6;
 inline bool step_shape_built(int shape, int q) {
   return shape == 0 || shape == 6 || q == 8 || q == 14;
 }
+// Workgroup size of the matrix-core step (many-component concentrations,
+// every degree): shape 5's 512 threads.
+constexpr int kMfmaStepShape = 5;
 
 struct StepArgs {
   int32_t shape;      // index into kStepShapeBlock
+  int32_t mfma;       // concentration components on the matrix cores
   int64_t npts;       // L * P arrival points
   int64_t nblocks;    // logical workgroups
   int64_t per_xcd;    // logical workgroups per XCD

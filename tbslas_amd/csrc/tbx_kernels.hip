@@ -6,6 +6,7 @@
 #include <type_traits>
 
 #include "tbx_device.h"
+#include "tbx_mfma.h"
 
 #ifndef TBX_Q
 #error "compile with -DTBX_Q=<degree>"
@@ -138,11 +139,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
 // thread that holds a point after a regroup carries it on: the RK2 state
 // (start point, wrapped midpoint) travels with the point through LDS, and
 // the leaf found for the regroup is the one the evaluation uses.  Lanes
-// past the last arrival point carry a copy of it and store nothing.
-template <int MODE, int KIND, int BLOCK, int WPE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_step_rg(
-    VelSource vel, TreeView conc, StepArgs a, double *__restrict__ vals,
-    double *__restrict__ dep) {
+// past the last arrival point carry a copy of it and store nothing.  MF:
+// the concentration's components go through eval_dof_mfma.
+template <int MODE, int KIND, int BLOCK, bool MF>
+__device__ __forceinline__ void k_step_rg_body(const VelSource &vel, const TreeView &conc,
+                                               const StepArgs &a, double *__restrict__ vals,
+                                               double *__restrict__ dep) {
   __shared__ RegroupLds<BLOCK, 6> sh;
   const uint32_t b = blockIdx.x;
   const int64_t lb = (int64_t)(b & 7u) * a.per_xcd + (int64_t)(b >> 3);
@@ -215,14 +217,36 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
     }
     const int64_t al = gi / a.P;
     const int am = (int)(gi - al * a.P);
-    double *o = vals + al * conc.dof * a.P + am;
-    eval_dof<D, MODE>(
-        conc, p3[0], p3[1], p3[2],
-        [&](int l, double v) {
-          if (live) o[(int64_t)l * a.P] = v;
-        },
-        lf);
+    if constexpr (MF) {
+      using T = typename ModeType<MODE>::T;
+      eval_dof_mfma<D, T>(conc, lf, p3[0], p3[1], p3[2], vals, al * conc.dof * a.P + am, a.P,
+                          live);
+    } else {
+      double *o = vals + al * conc.dof * a.P + am;
+      eval_dof<D, MODE>(
+          conc, p3[0], p3[1], p3[2],
+          [&](int l, double v) {
+            if (live) o[(int64_t)l * a.P] = v;
+          },
+          lf);
+    }
   }
+}
+
+template <int MODE, int KIND, int BLOCK, int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_step_rg(
+    VelSource vel, TreeView conc, StepArgs a, double *__restrict__ vals,
+    double *__restrict__ dep) {
+  k_step_rg_body<MODE, KIND, BLOCK, false>(vel, conc, a, vals, dep);
+}
+
+// The same step with the concentration's components contracted on the
+// matrix cores (conc.dof a multiple of 16; FP64 or FP32 arithmetic).
+template <int MODE, int KIND, int BLOCK, int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_step_rg_mfma(
+    VelSource vel, TreeView conc, StepArgs a, double *__restrict__ vals,
+    double *__restrict__ dep) {
+  k_step_rg_body<MODE, KIND, BLOCK, true>(vel, conc, a, vals, dep);
 }
 
 inline unsigned grid_for(int64_t n) {
@@ -280,6 +304,18 @@ hipError_t launch_step_rg_shape(const VelSource &vel, const TreeView &conc, cons
 template <int MODE>
 hipError_t launch_step_mode(const VelSource &vel, const TreeView &conc, const StepArgs &a,
                             double *vals, double *dep, hipStream_t s) {
+  // many components on the matrix cores: 512-thread workgroups, two waves
+  // per SIMD (the tile operands and accumulators need the registers)
+  if (a.mfma) {
+    if constexpr (MODE == kFma64 || MODE == kFp32) {
+      if (vel.kind != kVelTree || a.shape != kMfmaStepShape) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_step_rg_mfma<MODE, kVelTree, 512, 2>), dim3((unsigned)(8 * a.per_xcd)),
+                         dim3(512), 0, s, vel, conc, a, vals, dep);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   // the composite velocity kinds run the default shape only (the host
   // sizes the grid for the shape it asks for: kDefaultStepShape then)
   if (vel.kind == kVelExtrap)

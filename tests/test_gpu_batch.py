@@ -118,3 +118,47 @@ def test_batch_matches_per_problem_steps(torch, tb):
     b.step(3)
     for p in range(len(probs)):
         assert np.array_equal(b.coeffs(p), results[0][p])
+
+
+@pytest.mark.parametrize("q,dof,bc", [(8, 16, 0), (6, 48, 1), (5, 80, 0), (14, 32, 1)])
+def test_tracers_on_matrix_cores(torch, tb, orc, q, dof, bc):
+    # components in multiples of 16 run on the matrix cores (TBX_FP64 and
+    # TBX_FP32): FP64 against TBX_EXACT (the oracle's arithmetic, vector
+    # path) and against the vector contraction within 1e-13 (another
+    # summation order); FP32 against the FP32 vector contraction (the same
+    # departure points) within 5e-6: two FP32 summation orders over up to
+    # 680 terms of a slowly decaying random expansion (2.3e-6 measured at
+    # q = 14, below 1e-6 at q <= 8).  The concentration is random per leaf
+    # (discontinuous across faces), so FP32 is not compared with FP64: an
+    # FP32 departure point within 1e-8 of a face may take the other leaf.
+    # The velocity drifts (no stagnation on leaf faces).  A random adaptive
+    # tree puts several leaves in many waves.
+    from conftest import random_tree
+    from tbslas_amd import _lib, fields
+
+    anchor, depth = random_tree(90 + dof, max_depth=3)
+    drift = np.array([0.31, 0.17, 0.23])
+    vcf = tb.fit_function(lambda p: fields.taylor_green_2d3(p) + drift, anchor, depth, q, 3)
+    rng = np.random.default_rng(dof)
+    ccf = rng.standard_normal((len(depth), dof, tb.ncoef(q))) / (1.0 + np.arange(tb.ncoef(q)))
+    vel, conc = tb.Tree(anchor, depth, q, 3, vcf), tb.Tree(anchor, depth, q, dof, ccf)
+    dt = 0.02
+    ref = tb.semilag_values(vel, conc, 2, dt, 1, bc, "exact").cpu().numpy()
+    scale = np.abs(ref).max()
+    got = {}
+    try:
+        for sw in (1, 0):
+            _lib.set_option(_lib.OPT_TRACER_MFMA, sw)
+            for mode in ("fp64", "fp32"):
+                got[(sw, mode)] = tb.semilag_values(vel, conc, 2, dt, 1, bc, mode).cpu().numpy()
+    finally:
+        _lib.set_option(_lib.OPT_TRACER_MFMA, 1)
+    assert np.abs(got[(1, "fp64")] - ref).max() <= 1e-13 * scale
+    assert np.abs(got[(1, "fp64")] - got[(0, "fp64")]).max() <= 1e-13 * scale
+    assert np.abs(got[(1, "fp32")] - got[(0, "fp32")]).max() <= 5e-6 * scale
+    # the in-situ step (refit of every component) through the same path
+    o = tb.Tree(anchor, depth, q, dof, ccf)
+    tb.semilag_insitu(vel, o, 2, dt, 1, bc, "fp64")
+    e = tb.Tree(anchor, depth, q, dof, ccf)
+    tb.semilag_insitu(vel, e, 2, dt, 1, bc, "exact")
+    assert np.abs(o.get_coeffs() - e.get_coeffs()).max() <= 1e-12 * np.abs(e.get_coeffs()).max()
