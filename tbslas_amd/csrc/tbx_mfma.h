@@ -9,12 +9,11 @@
 //
 //   M = 16 components, N = 16 points, K = 4 basis terms per instruction.
 //
-// K walks the triangle row by row: for each (i, j) the x-degrees k of the
-// row go in chunks of four, the four lane groups g = lane / 16 taking
-// k = 4c + g (rows padded to a multiple of 4: 244 of 165 terms at q = 8).
-// A lane's B operand is then T_i(z) T_j(y) T_{4c+g}(x) of its tile point,
-// and its A operand coeff[c0 + lane % 16][base(i, j) + 4c + g], so only the
-// x-degree depends on the lane.
+// K walks the packed triangle four terms at a time, the four lane groups
+// g = lane / 16 taking terms 4q + g (165 terms in 42 chunks at q = 8).  A
+// lane's A operand is coeff[c0 + lane % 16][4q + g], its B operand
+// T_i(z) T_j(y) T_k(x) of term 4q + g at its tile point: the chunk's four
+// products are formed and the lane picks its own.
 //
 // Operand layouts of v_mfma_{f64,f32}_16x16x4 (both measured on gfx950):
 // A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k; the result D[i][j]
@@ -30,9 +29,22 @@
 // (four M tiles).
 #pragma once
 
+#include <utility>
+
 #include "tbx_device.h"
 
 namespace tbx {
+
+// f(integral_constant<int, I>) for I = 0 .. N - 1, unrolled at compile time
+// (the chunk loops are too long for the unroller's limits at high degree).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 template <typename T>
 struct MfmaAcc;
@@ -54,6 +66,13 @@ struct MfmaAcc<float> {
   static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
+__device__ __forceinline__ double fma_t(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+__device__ __forceinline__ float fma_t(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
 template <typename T>
 __device__ __forceinline__ T lane_read(T v, int src) {
   if constexpr (sizeof(T) == 8) {
@@ -72,34 +91,25 @@ __device__ __forceinline__ int64_t lane_read_i64(int64_t v, int src) {
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// The K chunks of the packed triangle in order: row (i, j) of length
-// R = D - i - j, chunk c covers x-degrees 4c .. 4c + 3 of the row, packed
-// index base + 4c + g for lane group g.
+// The packed triangle in chunks of four consecutive terms: chunk q holds
+// packed indices 4q .. 4q + 3 (lane group g takes 4q + g), with their
+// (i, j, k); indices past Nc are padding.
 template <int D>
 struct MfmaChunks {
-  static constexpr int count() {
-    int n = 0;
-    for (int i = 0; i < D; ++i)
-      for (int j = 0; i + j < D; ++j) n += (D - i - j + 3) / 4;
-    return n;
-  }
-  static constexpr int N = count();
-  int i[N], j[N], c[N], base[N], R[N];
-  constexpr MfmaChunks() : i(), j(), c(), base(), R() {
-    int n = 0, b = 0;
+  static constexpr int NC = D * (D + 1) * (D + 2) / 6;
+  static constexpr int N = (NC + 3) / 4;
+  int i[4 * N], j[4 * N], k[4 * N];
+  constexpr MfmaChunks() : i(), j(), k() {
+    int m = 0;
     for (int a = 0; a < D; ++a)
-      for (int bb = 0; a + bb < D; ++bb) {
-        const int r = D - a - bb;
-        for (int cc = 0; 4 * cc < r; ++cc) {
-          i[n] = a;
-          j[n] = bb;
-          c[n] = cc;
-          base[n] = b;
-          R[n] = r;
-          ++n;
+      for (int b = 0; a + b < D; ++b)
+        for (int c = 0; a + b + c < D; ++c) {
+          i[m] = a;
+          j[m] = b;
+          k[m] = c;
+          ++m;
         }
-        b += r;
-      }
+    for (; m < 4 * N; ++m) i[m] = j[m] = k[m] = 0;
   }
 };
 
@@ -122,46 +132,59 @@ __device__ __forceinline__ T buffer_coeff(__amdgpu_buffer_rsrc_t r, int voff, in
 // One leaf's terms for NT M tiles (16 NT components) and the two N tiles:
 // acc[mt][n] += coeff tile mt x basis tile n.  rsrc covers the leaf's
 // components from the first of the tiles; voff = the lane's byte offset
-// (component lane % 16, x-degree slot g); in[n]: the lane's tile point n
-// lies in the leaf (its B operand is zero otherwise).  The chunk table is
-// unrolled; the A operands are loaded two chunks ahead and a scheduling
-// barrier closes every chunk, which keeps the loads in flight under the
-// MFMAs without hoisting them all to the top.
+// (component lane % 16, term slot g); in[n]: the lane's tile point n lies
+// in the leaf (its B operand is zero otherwise).  K runs over the packed
+// triangle four terms at a time; the lane's B operand is the product of
+// its term's three basis values, picked among the chunk's four by 0/1
+// weights (no divergent selection).  The
+// chunks are unrolled; the A operands are loaded two chunks ahead and a
+// scheduling barrier closes every chunk, which keeps the loads in flight
+// under the MFMAs without hoisting them all to the top.
 template <int D, typename T, int NT>
 __device__ __forceinline__ void mfma_leaf_terms(__amdgpu_buffer_rsrc_t rsrc, int voff, int g,
                                                 const bool (&in)[2], const T (&tz)[2][D],
-                                                const T (&ty)[2][D],
-                                                const T (&txg)[2][(D + 3) / 4],
+                                                const T (&ty)[2][D], const T (&tx)[2][D],
                                                 typename MfmaAcc<T>::V (&acc)[4][2]) {
   using Acc = MfmaAcc<T>;
-  constexpr int NC = D * (D + 1) * (D + 2) / 6;
-  constexpr MfmaChunks<D> ch{};
+  constexpr int NC = MfmaChunks<D>::NC;
   constexpr int NQ = MfmaChunks<D>::N;
-  auto valid_at = [&](int q) { return 4 * ch.c[q] + 3 < ch.R[q] || 4 * ch.c[q] + g < ch.R[q]; };
+  const bool last_valid = 4 * (NQ - 1) + g < NC;  // the final chunk's padding
+  // the lane's pick among a chunk's four products as 0/1 weights: a
+  // branch-free, exact selection (finite products times 0 and 1)
+  T w[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) w[s] = g == s ? (T)1 : (T)0;
   auto load = [&](int q, T (&a)[NT]) {
-    const bool valid = valid_at(q);
 #pragma unroll
     for (int mt = 0; mt < NT; ++mt) {
-      const int soff = (int)sizeof(T) * (mt * 16 * NC + ch.base[q] + 4 * ch.c[q]);
+      const int soff = (int)sizeof(T) * (mt * 16 * NC + 4 * q);
       const T v = buffer_coeff<T>(rsrc, voff, soff);
-      a[mt] = valid ? v : (T)0;
+      a[mt] = (q < NQ - 1 || last_valid) ? v : (T)0;
     }
   };
   T a0[NT], a1[NT];
   load(0, a0);
   if (NQ > 1) load(1, a1);
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
+  static_for<NQ>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
     T a2[NT];
-    if (q + 2 < NQ) load(q + 2, a2);
-    const int i = ch.i[q], j = ch.j[q], c = ch.c[q];
-    const bool valid = valid_at(q);
+    if constexpr (q + 2 < NQ) load(q + 2, a2);
     T bv[2];
 #pragma unroll
     for (int n = 0; n < 2; ++n) {
       // the leaf mask first: products of the pass, not hoisted out of it
-      const T zm = in[n] ? tz[n][i] : (T)0;
-      bv[n] = valid ? zm * ty[n][j] * txg[n][c] : (T)0;
+      T p[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        constexpr MfmaChunks<D> tab{};
+        const int m = 4 * q + s;
+        const T zm = in[n] ? tz[n][tab.i[m]] : (T)0;
+        p[s] = m < NC ? zm * ty[n][tab.j[m]] * tx[n][tab.k[m]] : (T)0;
+      }
+      T b = p[0] * w[0];
+#pragma unroll
+      for (int s = 1; s < 4; ++s) b = fma_t(p[s], w[s], b);
+      bv[n] = b;
     }
 #pragma unroll
     for (int mt = 0; mt < NT; ++mt)
@@ -170,10 +193,10 @@ __device__ __forceinline__ void mfma_leaf_terms(__amdgpu_buffer_rsrc_t rsrc, int
 #pragma unroll
     for (int mt = 0; mt < NT; ++mt) {
       a0[mt] = a1[mt];
-      if (q + 2 < NQ) a1[mt] = a2[mt];
+      if constexpr (q + 2 < NQ) a1[mt] = a2[mt];
     }
     __builtin_amdgcn_sched_barrier(0);
-  }
+  });
 }
 
 // Values of all t.dof components (dof a multiple of 16) at the wave's
@@ -186,7 +209,6 @@ __device__ __forceinline__ void eval_dof_mfma(const TreeView &t, int leaf, doubl
                                               int64_t cstride, bool live) {
   using Acc = MfmaAcc<T>;
   using V = typename Acc::V;
-  constexpr int NCH = (D + 3) / 4;  // chunks of the longest row
   constexpr int NC = D * (D + 1) * (D + 2) / 6;
   const int lane = (int)(threadIdx.x & 63u);
   const int g = lane >> 4, col = lane & 15;
@@ -201,7 +223,7 @@ __device__ __forceinline__ void eval_dof_mfma(const TreeView &t, int leaf, doubl
 #pragma unroll 1
   for (int half = 0; half < 2; ++half) {
     // tile points of this lane: 32 half + 16 n + col
-    T tz[2][D], ty[2][D], txg[2][NCH];
+    T tz[2][D], ty[2][D], tx[2][D];
     int pleaf[2];
     int64_t pbase[2];
     bool plive[2];
@@ -213,15 +235,7 @@ __device__ __forceinline__ void eval_dof_mfma(const TreeView &t, int leaf, doubl
       pleaf[n] = __builtin_amdgcn_ds_bpermute(src << 2, leaf);
       pbase[n] = lane_read_i64(obase, src);
       plive[n] = __builtin_amdgcn_ds_bpermute(src << 2, live ? 1 : 0) != 0;
-      T tx[4 * NCH];
-      cheb_poly<D>(px, tx);
-#pragma unroll
-      for (int k = D; k < 4 * NCH; ++k) tx[k] = (T)0;
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        const T a0 = tx[4 * c], a1 = tx[4 * c + 1], a2 = tx[4 * c + 2], a3 = tx[4 * c + 3];
-        txg[n][c] = g == 0 ? a0 : g == 1 ? a1 : g == 2 ? a2 : a3;
-      }
+      cheb_poly<D>(px, tx[n]);
       cheb_poly<D>(py, ty[n]);
       cheb_poly<D>(pz, tz[n]);
     }
@@ -246,16 +260,16 @@ __device__ __forceinline__ void eval_dof_mfma(const TreeView &t, int leaf, doubl
         const int voff = (int)sizeof(T) * (col * NC + g);
         switch (ntile) {  // wave-uniform
           case 4:
-            mfma_leaf_terms<D, T, 4>(rsrc, voff, g, in, tz, ty, txg, acc);
+            mfma_leaf_terms<D, T, 4>(rsrc, voff, g, in, tz, ty, tx, acc);
             break;
           case 3:
-            mfma_leaf_terms<D, T, 3>(rsrc, voff, g, in, tz, ty, txg, acc);
+            mfma_leaf_terms<D, T, 3>(rsrc, voff, g, in, tz, ty, tx, acc);
             break;
           case 2:
-            mfma_leaf_terms<D, T, 2>(rsrc, voff, g, in, tz, ty, txg, acc);
+            mfma_leaf_terms<D, T, 2>(rsrc, voff, g, in, tz, ty, tx, acc);
             break;
           default:
-            mfma_leaf_terms<D, T, 1>(rsrc, voff, g, in, tz, ty, txg, acc);
+            mfma_leaf_terms<D, T, 1>(rsrc, voff, g, in, tz, ty, tx, acc);
             break;
         }
       }
