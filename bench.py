@@ -40,6 +40,7 @@ SEED_DEPTH, MAX_DEPTH, TOL = 5, 7, 1e-13
 SIGMA, CENTRE = 0.04, (0.7, 0.5, 0.5)
 DT = 1e-3
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X published dense
+MFMA_PEAK_F64, MFMA_PEAK_F32 = PEAK["fp64"], PEAK["fp32"]  # dense matrix cores
 HBM_PEAK = 8000.0  # GB/s
 
 
@@ -226,6 +227,12 @@ def c4_measure():
             "adjoint_run_over_plain_forward": run_ms / fwd_ms}
 
 
+def _lib_option(name: str) -> int:
+    from tbslas_amd import _lib
+
+    return _lib.get_option(getattr(_lib, name))
+
+
 def c3a_measure(steps: int = 2):
     """Batched tracers at full size: 64 seeded Gaussian tracers (SplitMix64
     seed 3) on a uniform depth-6 tree (262,144 leaves), q = 8, carried by one
@@ -279,8 +286,19 @@ def c3a_measure(steps: int = 2):
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / steps
+        # the tracer contraction runs on the matrix cores: its algorithmic
+        # flops (2 T Nc per departure point) over the whole step time, a
+        # lower bound of the kernel's own fraction (the step also holds the
+        # trajectories and the FP64 refit GEMM)
+        flops = 2.0 * T * tb.ncoef(q) * N
+        peak = MFMA_PEAK_F64 if mode == "fp64" else MFMA_PEAK_F32
         out[mode] = {"ms_per_step": ms, "interps_per_s": N * (2 + T) / (ms * 1e-3),
-                     "departure_points_per_s": N / (ms * 1e-3)}
+                     "departure_points_per_s": N / (ms * 1e-3),
+                     "tracer_contraction_tflops_over_step": flops / (ms * 1e-3) / 1e12,
+                     "mfma_peak_tflops": peak,
+                     "mfma_frac_over_step": flops / (ms * 1e-3) / 1e12 / peak}
+    out["tracer_contraction"] = ("matrix cores" if _lib_option("OPT_TRACER_MFMA") else
+                                 "vector ALU")
     out["linf_rel_fp32_vs_fp64_after_1_step"] = float(
         (res["fp32"] - res["fp64"]).abs().max() / res["fp64"].abs().max())
     out["fp32_over_fp64"] = out["fp32"]["interps_per_s"] / out["fp64"]["interps_per_s"]
@@ -390,6 +408,8 @@ def main():
                     help="also measure the 256-problem persistent batch (about 200 GB of "
                          "device memory, a few minutes)")
     ap.add_argument("--c3b-only", action="store_true", help="only the 256-problem batch")
+    ap.add_argument("--c3a-only", action="store_true", help="only the 64-tracer measurement")
+    ap.add_argument("--c4-only", action="store_true", help="only the adjoint measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
@@ -402,6 +422,12 @@ def main():
 
     if args.c3b_only:
         print(json.dumps({"c3b_batch": c3b_measure(2)}))
+        return
+    if args.c3a_only:
+        print(json.dumps({"c3a_64_tracers": c3a_measure(2)}))
+        return
+    if args.c4_only:
+        print(json.dumps({"c4_adjoint": c4_measure()}))
         return
     t_setup = time.perf_counter()
     anchor, depth, ccf, vcf = build_workload()
