@@ -382,11 +382,16 @@ __device__ __forceinline__ void eval_vec3(const TreeView &t, double x,
                                           double &v1, double &v2, int hint = -1) {
   PointBasis<D, MODE> b;
   b.build(t, x, y, z, hint);
+  // the waves of a workgroup start at different components: after a
+  // regrouping they reach the same leaf together, and in the same order
+  // they would all miss on the same coefficient lines at once
+  const int l0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) % 3;
   for_each_wave_leaf(b.leaf, [&](int lf) {
     // one component at a time: unrolled side by side, the three
     // contractions' scalar loads exhaust the SGPRs and spill
 #pragma unroll 1
-    for (int l = 0; l < 3; ++l) {
+    for (int k = 0; k < 3; ++k) {
+      const int l = l0 + k < 3 ? l0 + k : l0 + k - 3;
       const double u = leaf_value<D, MODE>(t, lf, l, b);
       v0 = l == 0 ? u : v0;
       v1 = l == 1 ? u : v1;
