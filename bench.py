@@ -13,7 +13,7 @@ the velocity tree -> concentration at the departure points -> refit.
 Reported: whole-job departure-point interpolations/s (3 per point per step:
 two velocity evaluations and one concentration evaluation) of three
 arithmetic modes of the same step: plain FP64, FP32 (coefficients and
-accumulation in FP32) and FP32 with the degree <= 2 head of each
+accumulation in FP32) and FP32 with the degree <= 1 head of each
 contraction in FP64 (headline `value`); the L-inf distance of each FP32
 mode from the FP64 path after the first step and after all steps; the
 roofline of the dominant kernel measured with HIP events on the launch
@@ -531,7 +531,7 @@ def main():
             "leaves": L, "q": Q, "departure_points_per_step": N, "interps_per_step": 3 * N,
             "dt": DT, "bc": "periodic",
         },
-        "headline_mode": "fp32h (FP32 contraction, degree <= 2 head in FP64)",
+        "headline_mode": "fp32h (FP32 contraction, degree <= 1 head in FP64)",
         "fp64": res["fp64"],
         "fp32": res["fp32"],
         "fp32h": res["fp32h"],

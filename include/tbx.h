@@ -61,8 +61,8 @@ enum { TBX_BC_FREESPACE = 0, TBX_BC_PERIODIC = 1 }; /* sim_config.h:59 bc */
                        /* bit-identical to the CPU path                  */
 #define TBX_FP32 0x2u  /* FP32 coefficients and accumulation; point      */
                        /* coordinates, RK2 updates and keys stay FP64    */
-#define TBX_FP32_HEAD64 0x3u /* TBX_FP32 for the terms of total degree > 2, */
-                             /* FP64 for the 10 terms of degree <= 2, which */
+#define TBX_FP32_HEAD64 0x3u /* TBX_FP32 for the terms of total degree > 1, */
+                             /* FP64 for the 4 terms of degree <= 1, which  */
                              /* carry most of a smooth field's magnitude    */
 #define TBX_MODE_MASK 0xFu
 /* EvalTree wraps the caller's points in place under a periodic bc

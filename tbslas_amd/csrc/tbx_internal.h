@@ -17,7 +17,7 @@ constexpr double kKeyScale = 2097152.0;  // 2^21
 // kFp32h: FP32 tail of the contraction (total degree > kHeadDeg) plus the
 // head (total degree <= kHeadDeg) in FP64.
 enum Mode : int { kFma64 = 0, kExact64 = 1, kFp32 = 2, kFp32h = 3 };
-constexpr int kHeadDeg = 2;
+constexpr int kHeadDeg = 1;
 
 // Device view of one tree.  All pointers are device memory owned by the
 // tree handle.

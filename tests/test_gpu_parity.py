@@ -376,7 +376,7 @@ def test_every_step_shape_matches_oracle(torch, tb, orc, q):
 
 def test_fp32_head64_mode(torch, tb, orc):
     # coefficients decaying 10x per total degree, like a well-resolved field:
-    # the FP64 head leaves only the FP32 rounding of the degree > 2 tail,
+    # the FP64 head leaves only the FP32 rounding of the degree > 1 tail,
     # bounded by a few FP32 ulps of the tail's coefficient sum; the pure
     # FP32 mode rounds the whole value
     q = 10
@@ -391,10 +391,12 @@ def test_fp32_head64_mode(torch, tb, orc):
     scale = np.abs(ref).max()
     e32 = np.abs(g.eval(dev(torch, x), 0, "fp32").cpu().numpy() - ref).max() / scale
     e32h = np.abs(g.eval(dev(torch, x), 0, "fp32h").cpu().numpy() - ref).max() / scale
-    tail = np.abs(coeff[..., deg > 2]).sum(-1).max() / scale
+    tail = np.abs(coeff[..., deg > 1]).sum(-1).max() / scale
     assert e32 <= FP32_TOL
     assert e32h <= 8 * 2.0 ** -24 * tail + 1e-14, (e32h, tail)
-    assert e32h * 100 <= e32, (e32, e32h)
+    # the tail is 3 % of the coefficient sum here: the head removes most of
+    # the FP32 rounding (about 30x; 10x asserted)
+    assert e32h * 10 <= e32, (e32, e32h)
     # slowly decaying coefficients still meet the FP32 bar
     coeff = rng.standard_normal(coeff.shape) / (1 + np.arange(coeff.shape[-1]))
     g.set_coeffs(coeff)
