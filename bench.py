@@ -172,6 +172,31 @@ def c5_measure(steps: int):
     return out
 
 
+def locate_measure(tree, n: int = 1 << 27, reps: int = 5):
+    """Bit-exact leaf location alone (tbx_locate: periodic wrap, 21-bit
+    keys, directory and search) of n uniformly random points on the
+    benchmark tree: HBM-bound, algorithmic bytes 24 (coordinates) + 4 (leaf
+    id) per point."""
+    import torch
+
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    pts = torch.rand((n, 3), dtype=torch.float64, device="cuda", generator=gen)
+    tree.locate(pts, 1)  # warm-up
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        tree.locate(pts, 1)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 28 * n
+    del pts
+    return {"points": n, "ms": ms, "points_per_s": n / (ms * 1e-3),
+            "algorithmic_bytes": nbytes, "hbm_gbs": nbytes / (ms * 1e-3) / 1e9,
+            "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK}
+
+
 def c4_measure():
     """The FP64 adjoint at full size: uniform depth 5 (32,768 leaves), q = 8,
     stationary random Chebyshev velocity (SplitMix64 seed 4, |v| <= 1), a
@@ -543,6 +568,7 @@ def main():
         "roofline_fp32": roof("fp32"),
         "roofline_fp64": roof("fp64"),
     }
+    out["locate"] = locate_measure(vel)
     if not args.no_c5:
         del trees, vel
         torch.cuda.empty_cache()

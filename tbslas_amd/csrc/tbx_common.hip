@@ -8,20 +8,46 @@ namespace tbx {
 namespace {
 
 // EvalTree's wrap + keys + leaf assignment (tree_functor.h:442-489,
-// :190-198) without evaluation; pts are left untouched.
+// :190-198) without evaluation; pts are left untouched.  A block takes
+// kLocatePts x kBlock consecutive points: their coordinates (24 bytes a
+// point) come in through LDS with 16-byte loads, consecutive per lane, and
+// each thread then locates kLocatePts points a block apart.
+constexpr int kLocatePts = 4;
+
 __global__ __launch_bounds__(kBlock) void k_locate(TreeView t,
                                                    const double *__restrict__ pts,
                                                    int64_t n, int bc,
                                                    int32_t *__restrict__ leaf) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  double x = pts[3 * i + 0], y = pts[3 * i + 1], z = pts[3 * i + 2];
-  if (bc == TBX_BC_PERIODIC) {
-    x = wrap1(x);
-    y = wrap1(y);
-    z = wrap1(z);
+  constexpr int NP = kBlock * kLocatePts;  // points of the block
+  __shared__ double4 stage[NP * 3 / 4];    // NP x 3 doubles
+  const int64_t p0 = (int64_t)blockIdx.x * NP;
+  const int64_t np = n - p0 < NP ? n - p0 : NP;
+  const int tid = threadIdx.x;
+  if (np == NP && (reinterpret_cast<uintptr_t>(pts) & 15u) == 0) {
+    // the block's 3 NP doubles as NP * 3 / 2 16-byte pieces
+    const double2 *src = reinterpret_cast<const double2 *>(pts + 3 * p0);
+    double2 *dst = reinterpret_cast<double2 *>(stage);
+#pragma unroll
+    for (int k = 0; k < NP * 3 / 2 / kBlock; ++k) dst[k * kBlock + tid] = src[k * kBlock + tid];
+  } else {
+    double *dst = reinterpret_cast<double *>(stage);
+    for (int k = tid; k < 3 * np; k += kBlock) dst[k] = pts[3 * p0 + k];
   }
-  leaf[i] = locate(t, x, y, z);
+  __syncthreads();
+  const double *c = reinterpret_cast<const double *>(stage);
+#pragma unroll
+  for (int k = 0; k < kLocatePts; ++k) {
+    const int j = k * kBlock + tid;
+    if (j < np) {
+      double a = c[3 * j + 0], b = c[3 * j + 1], z = c[3 * j + 2];
+      if (bc == TBX_BC_PERIODIC) {
+        a = wrap1(a);
+        b = wrap1(b);
+        z = wrap1(z);
+      }
+      leaf[p0 + j] = locate(t, a, b, z);
+    }
+  }
 }
 
 struct NodeTable {
@@ -62,8 +88,9 @@ inline unsigned grid_for(int64_t n) {
 hipError_t launch_locate(const TreeView &t, const double *pts, int64_t n,
                          int bc, int32_t *leaf, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_locate, dim3(grid_for(n)), dim3(kBlock), 0, s, t, pts,
-                     n, bc, leaf);
+  const int64_t per = (int64_t)kBlock * kLocatePts;
+  hipLaunchKernelGGL(k_locate, dim3((unsigned)((n + per - 1) / per)), dim3(kBlock), 0, s, t,
+                     pts, n, bc, leaf);
   return hipGetLastError();
 }
 
