@@ -201,9 +201,13 @@ def c4_measure():
     """The FP64 adjoint at full size: uniform depth 5 (32,768 leaves), q = 8,
     stationary random Chebyshev velocity (SplitMix64 seed 4, |v| <= 1), a
     Gaussian blob, 20 steps of dt = 0.01, FreeSpace, checkpoints every 5
-    steps.  The adjoint run = 20 forward steps + 16 recomputed + 20 adjoint
-    steps + one velocity scatter; the adjoint step is the run less its 36
-    forward steps, per step (the velocity scatter spread over the 20)."""
+    steps.  The adjoint run = 20 forward steps + 16 replayed + 20 adjoint
+    steps + one velocity scatter.  With the stationary velocity the run keeps
+    step 1's departure points and its other 35 forward steps evaluate the
+    concentration there and refit; the adjoint step is the run less its
+    forward steps (one full step + 35 of those, timed on their own), per
+    step.  Also reported: one tbx_semilag_insitu_vjp reading back the
+    forward's departure points, without and with the velocity cotangent."""
     import torch
 
     import tbslas_amd as tb
@@ -240,14 +244,29 @@ def c4_measure():
 
     fwd_ms = timed(forward, 3)
     run_ms = timed(run, 3)
-    recomputed = T - (T + K - 1) // K
     step_ms = fwd_ms / T
-    adj_ms = (run_ms - (T + recomputed) * step_ms) / T
+    from tbslas_amd.adjoint import semilag_insitu_vjp
+
+    lam = torch.randn(c0.shape, dtype=torch.float64, device="cuda",
+                      generator=torch.Generator(device="cuda").manual_seed(5))
+    gv = torch.zeros((vel.L, 3, vel.nc), dtype=torch.float64, device="cuda")
+    dep = torch.empty((conc.L * conc.P, 3), dtype=torch.float64, device="cuda")
+    conc.set_coeffs(c0d)
+    tb.semilag_insitu(vel, conc, 1, dt, 1, 0, "fp64", dep=dep)
+    conc.set_coeffs(c0d)
+    vjp_c = timed(lambda: semilag_insitu_vjp(vel, conc, 1, dt, lam, 0, "fp64", dep=dep), 5)
+    vjp_v = timed(lambda: semilag_insitu_vjp(vel, conc, 1, dt, lam, 0, "fp64", grad_vel=gv,
+                                             dep=dep), 3)
+    conc.set_coeffs(c0d)
+    short_ms = timed(lambda: conc.set_grid_values(conc.eval(dep, 0, "fp64")), 5)
+    replays = T - (T + K - 1) // K
+    adj_ms = (run_ms - step_ms - (T - 1 + replays) * short_ms) / T
     del vel, conc
     return {"leaves": len(depth), "q": q, "steps": T, "checkpoint_every": K,
             "grid_points": len(depth) * (q + 1) ** 3,
             "forward_step_ms": step_ms, "adjoint_run_ms": run_ms,
-            "plain_forward_ms": fwd_ms, "adjoint_step_ms": adj_ms,
+            "plain_forward_ms": fwd_ms, "stored_trajectory_step_ms": short_ms,
+            "vjp_ms": vjp_c, "vjp_with_velocity_ms": vjp_v, "adjoint_step_ms": adj_ms,
             "adjoint_step_over_forward_step": adj_ms / step_ms,
             "adjoint_run_over_plain_forward": run_ms / fwd_ms}
 

@@ -274,12 +274,14 @@ int tbx_batch_get_coeffs(const tbx_batch *b, int p, double *coeff, tbx_stream s)
 /* Cotangent of one tbx_semilag_insitu step taken from conc's current
  * coefficients c (left unchanged): lam_out = dL/dc given lam_next = dL/dc'
  * (device arrays [L][dof][Nc]); grad_vel (device [L_vel][3][Nc_vel], may be
- * NULL) is ADDED to.  num_rk_step must be 1. */
+ * NULL) is ADDED to.  dep (device [L*P][3], may be NULL): the departure
+ * points that forward step wrote (its dep output); read back instead of
+ * integrating the trajectories again.  num_rk_step must be 1. */
 int tbx_semilag_insitu_vjp(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
                            const tbx_tree *conc, int timestep, double dt,
                            int num_rk_step, int bc, uint32_t flags,
-                           const double *lam_next, double *lam_out,
-                           double *grad_vel, tbx_stream s);
+                           const double *dep, const double *lam_next,
+                           double *lam_out, double *grad_vel, tbx_stream s);
 
 /* nsteps in-situ steps from conc's coefficients c_0 (timesteps 1..nsteps),
  * loss = 0.5 |c_T - c_target|^2 (returned on the host), and its gradients:
@@ -290,8 +292,10 @@ int tbx_semilag_insitu_vjp(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
  * velocity, g NULL, whose trajectories are the same every step) are kept
  * on the device, 24 bytes per grid point and step, while they fit in half
  * the free device memory, and the adjoint steps read them instead of
- * integrating again.  A stationary velocity scatters its cotangent once.
- * conc holds c_T on return. */
+ * integrating again.  A stationary velocity scatters its cotangent once,
+ * and once its departure points are kept, its one-component forward steps
+ * (the first sweep and the replays) evaluate the concentration there and
+ * refit, with the same results as a full step.  conc holds c_T on return. */
 int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
                     tbx_tree *conc, int nsteps, double dt, int bc,
                     uint32_t flags, const double *c_target, int checkpoint_every,

@@ -101,7 +101,7 @@ _SIGS = {
     "tbx_semilag_insitu_v": (_I32, [C.POINTER(_Velocity), _P, _I32, _D, _I32, _I32, _U32, _P,
                                     _P]),
     "tbx_semilag_insitu_vjp": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P,
-                                      _P, _P]),
+                                      _P, _P, _P]),
     "tbx_adjoint_run": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _U32, _P, _I32, _P, _P,
                                C.POINTER(C.c_double), _P]),
     "tbx_grid_create": (_I32, [C.POINTER(_GridDesc), C.POINTER(_P)]),

@@ -16,10 +16,13 @@ from .tree import Tree, _mode, _ptr, _require_dev, _stream
 
 
 def semilag_insitu_vjp(vel: Tree, conc: Tree, timestep: int, dt: float, lam_next,
-                       bc: int = BC_FREESPACE, mode: str = "fp64", g=None, grad_vel=None):
+                       bc: int = BC_FREESPACE, mode: str = "fp64", g=None, grad_vel=None,
+                       dep=None):
     """Cotangent of one in-situ step taken from conc's current coefficients:
     returns dL/dc for lam_next = dL/dc' (cuda [L, dof, Nc]); grad_vel (cuda
-    [L_vel, 3, Nc]) is added to when given."""
+    [L_vel, 3, Nc]) is added to when given.  dep: the departure points the
+    forward step wrote (semilag_insitu(..., dep=)), read back instead of
+    integrating the trajectories again."""
     import torch
 
     _require_dev(lam_next, torch.float64, "lam_next")
@@ -29,10 +32,14 @@ def semilag_insitu_vjp(vel: Tree, conc: Tree, timestep: int, dt: float, lam_next
         _require_dev(grad_vel, torch.float64, "grad_vel")
         if grad_vel.numel() != vel.L * 3 * vel.nc:
             raise ValueError("grad_vel must hold L_vel*3*Nc values")
+    if dep is not None:
+        _require_dev(dep, torch.float64, "dep", (3,))
+        if dep.shape[0] != conc.L * conc.P:
+            raise ValueError("dep must hold the L*P departure points of the step")
     lam_out = torch.empty_like(lam_next)
     cb = tfac(g)
     check(lib().tbx_semilag_insitu_vjp(vel.handle, cb, None, conc.handle, timestep, dt, 1, bc,
-                                       _mode(mode), _ptr(lam_next), _ptr(lam_out),
+                                       _mode(mode), _ptr(dep), _ptr(lam_next), _ptr(lam_out),
                                        _ptr(grad_vel), _stream()))
     return lam_out
 
@@ -74,21 +81,23 @@ def _step_function():
 
             vel.set_coeffs(v.contiguous())
             conc.set_coeffs(c.contiguous())
-            semilag_insitu(vel, conc, timestep, dt, 1, bc, mode)
+            # the trajectories are kept for the backward pass
+            dep = torch.empty((conc.L * conc.P, 3), dtype=torch.float64, device="cuda")
+            semilag_insitu(vel, conc, timestep, dt, 1, bc, mode, dep=dep)
             out = conc.get_coeffs_device()
-            ctx.save_for_backward(c, v)
+            ctx.save_for_backward(c, v, dep)
             ctx.args = (vel, conc, timestep, dt, bc, mode)
             return out
 
         @staticmethod
         def backward(ctx, lam_next):
-            c, v = ctx.saved_tensors
+            c, v, dep = ctx.saved_tensors
             vel, conc, timestep, dt, bc, mode = ctx.args
             vel.set_coeffs(v.contiguous())
             conc.set_coeffs(c.contiguous())
             grad_v = torch.zeros_like(v) if ctx.needs_input_grad[1] else None
             lam = semilag_insitu_vjp(vel, conc, timestep, dt, lam_next.contiguous(), bc, mode,
-                                     grad_vel=grad_v)
+                                     grad_vel=grad_v, dep=dep)
             return lam, grad_v, None, None, None, None, None, None
 
     return SemilagInSituStep

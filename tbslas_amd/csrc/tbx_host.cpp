@@ -948,8 +948,8 @@ int ensure_adjA(tbx_tree *conc, hipStream_t st) {
 
 int tbx_semilag_insitu_vjp(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, const tbx_tree *concc,
                            int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
-                           const double *lam_next, double *lam_out, double *grad_vel,
-                           tbx_stream s) {
+                           const double *dep, const double *lam_next, double *lam_out,
+                           double *grad_vel, tbx_stream s) {
   tbx_tree *conc = const_cast<tbx_tree *>(concc);
   hipStream_t st = (hipStream_t)s;
   AdjCtx c;
@@ -959,7 +959,7 @@ int tbx_semilag_insitu_vjp(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, const
   if (!lam_next || !lam_out) return fail(TBX_EINVAL, "tbx_semilag_insitu_vjp: null cotangent");
   if (grad_vel && (rc = ensure_adjA(conc, st))) return rc;
   if ((rc = adj_conc_step(c, conc, timestep, dt, bc, lam_next, lam_out,
-                          grad_vel ? conc->d_adjA : nullptr, nullptr, st)))
+                          grad_vel ? conc->d_adjA : nullptr, dep, st)))
     return rc;
   if (grad_vel) return adj_vel_step(c, conc, timestep, dt, bc, conc->d_adjA, grad_vel, st);
   return TBX_OK;
@@ -1006,12 +1006,28 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
     HIP_TRY(hipMalloc(&lam_b, bytes));
     HIP_TRY(hipMalloc(&cT, bytes));
     const tbx_velocity &v = c.v;
+    // One forward step.  With a stationary velocity every step has step 1's
+    // departure points: once they are kept, a one-component step is the
+    // concentration at them plus the refit (the evaluation kernel's output
+    // layout is the step's value layout for one component, and its
+    // per-point arithmetic the step kernel's: the same coefficients).
+    bool have_dep = false;
+    auto forward = [&](int n, double *dep) -> int {
+      if (stationary && keep_dep && have_dep && conc->dof == 1) {
+        HIP_TRY(dispatch_eval(conc->q, c.mode, conc->view(), deps[0], (int64_t)conc->L * conc->P(),
+                              bc, 0, conc->d_vals, st));
+        return refit(conc, conc->d_vals, st);
+      }
+      const int r = step_insitu(&v, conc, n, dt, 1, bc, flags, dep, st, "tbx_adjoint_run");
+      if (!r && dep && stationary) have_dep = true;
+      return r;
+    };
     // forward sweep, keeping c_0, c_K, c_2K, ... (and, for a stationary
     // velocity, the trajectories of step 1)
     HIP_TRY(hipMemcpyAsync(ck[0], conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
     for (int n = 1; n <= nsteps; ++n) {
       double *dep = (stationary && keep_dep && n == 1) ? deps[0] : nullptr;
-      int r = step_insitu(&v, conc, n, dt, 1, bc, flags, dep, st, "tbx_adjoint_run");
+      int r = forward(n, dep);
       if (r) return r;
       if (n % K == 0 && n / K < nck)
         HIP_TRY(hipMemcpyAsync(ck[n / K], conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
@@ -1041,7 +1057,7 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
       conc->f32_stale = true;
       for (int n = s0 + 1; n < s1; ++n) {
         double *dep = (!stationary && keep_dep) ? deps[n - s0 - 1] : nullptr;
-        int r = step_insitu(&v, conc, n, dt, 1, bc, flags, dep, st, "tbx_adjoint_run");
+        int r = forward(n, dep);
         if (r) return r;
         HIP_TRY(hipMemcpyAsync(seg[n - s0], conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
       }

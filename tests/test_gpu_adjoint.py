@@ -240,3 +240,28 @@ def test_run_equals_chained_vjp(torch, tb, bc, timed, K):
     assert abs(loss - 0.5 * float(((states[-1] - target) ** 2).sum())) <= 1e-13 * loss
     assert float((gc - lam).abs().max()) <= 1e-12 * float(lam.abs().max())
     assert float((gvr - gv).abs().max()) <= 1e-11 * float(gv.abs().max())
+
+
+@pytest.mark.parametrize("bc,timed", [(0, False), (1, True)])
+def test_vjp_with_stored_trajectories(torch, tb, bc, timed):
+    # the departure points the forward step wrote, read back by the VJP,
+    # give the VJP that integrates them again (atomic sums: to rounding)
+    from tbslas_amd.adjoint import semilag_insitu_vjp
+
+    g = (lambda t: math.cos(math.pi * t)) if timed else None
+    _, _, vel, conc, c0 = _setup(tb, 65, 6, 2)
+    c = torch.from_numpy(c0).cuda()
+    dep = torch.empty((conc.L * conc.P, 3), dtype=torch.float64, device="cuda")
+    conc.set_coeffs(c)
+    tb.semilag_insitu(vel, conc, 3, 0.03, 1, bc, "fp64", g=g, dep=dep)
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    lam = torch.randn(c0.shape, dtype=torch.float64, device="cuda", generator=gen)
+    out = []
+    for d in (None, dep):
+        conc.set_coeffs(c)
+        gv = torch.zeros((vel.L, 3, vel.nc), dtype=torch.float64, device="cuda")
+        out.append((semilag_insitu_vjp(vel, conc, 3, 0.03, lam, bc, "fp64", g=g, grad_vel=gv,
+                                       dep=d), gv))
+    (l0, g0), (l1, g1) = out
+    assert float((l0 - l1).abs().max()) <= 1e-13 * float(l0.abs().max())
+    assert float((g0 - g1).abs().max()) <= 1e-13 * float(g0.abs().max())
