@@ -130,7 +130,7 @@ def cpu_baseline(anchor, depth, ccf, vcf, n_leaves: int):
     return 3 * npts / t, npts, n_leaves, t
 
 
-def c5_measure(steps: int):
+def tricubic_measure(steps: int):
     """The tricubic variant at full size: 262,144 leaves (uniform depth 6)
     of 8^3 samples, Lamb-Oseen pair velocity, smoothed slotted sphere,
     dt = 5e-4, FreeSpace.  Returns per-mode step times and the achieved HBM
@@ -197,7 +197,7 @@ def locate_measure(tree, n: int = 1 << 27, reps: int = 5):
             "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK}
 
 
-def c4_measure():
+def adjoint_measure():
     """The FP64 adjoint at full size: uniform depth 5 (32,768 leaves), q = 8,
     stationary random Chebyshev velocity (SplitMix64 seed 4, |v| <= 1), a
     Gaussian blob, 20 steps of dt = 0.01, FreeSpace, checkpoints every 5
@@ -277,7 +277,7 @@ def _lib_option(name: str) -> int:
     return _lib.get_option(getattr(_lib, name))
 
 
-def c3a_measure(steps: int = 2):
+def tracers_measure(steps: int = 2):
     """Batched tracers at full size: 64 seeded Gaussian tracers (SplitMix64
     seed 3) on a uniform depth-6 tree (262,144 leaves), q = 8, carried by one
     trajectory set of the rigid rotation v = 2 pi (0.5 - y, x - 0.5, 0),
@@ -351,7 +351,7 @@ def c3a_measure(steps: int = 2):
     return out
 
 
-def c3b_problems(n: int = 256, seed: int = 33):
+def batch_problems(n: int = 256, seed: int = 33):
     """The batch workload: n problems with depth uniform in {3, 4, 5, 6},
     degree in {6, 8, 10}, a rigid rotation (omega = 2 pi) about a uniformly
     random axis through the centre, a Gaussian blob (sigma 0.08 at (0.6,
@@ -369,8 +369,8 @@ def c3b_problems(n: int = 256, seed: int = 33):
     return out
 
 
-def c3b_measure(steps: int = 2):
-    """256 small problems (c3b_problems) in one persistent graph-captured
+def batch_measure(steps: int = 2):
+    """256 small problems (batch_problems) in one persistent graph-captured
     launch, FP32 contraction; the launch configuration autotuned."""
     import torch
 
@@ -381,7 +381,7 @@ def c3b_measure(steps: int = 2):
     t0 = time.perf_counter()
     vel, conc, dts = [], [], []
     npts = 0
-    for depth_u, q, axis in c3b_problems():
+    for depth_u, q, axis in batch_problems():
         anchor, depth = tb.uniform_leaves(depth_u)
         vel.append(tb.Tree(anchor, depth, q, 3,
                            fields.rotation_coefficients(anchor, depth, q, axis, 2 * math.pi)))
@@ -445,15 +445,15 @@ def main():
                          "for byte counters), then --steps step kernels of one "
                          "arithmetic mode (fp64, fp32, fp32h, exact); no JSON line")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-c5", action="store_true", help="skip the tricubic-variant measurement")
-    ap.add_argument("--no-c3a", action="store_true", help="skip the 64-tracer measurement")
-    ap.add_argument("--no-c4", action="store_true", help="skip the adjoint measurement")
-    ap.add_argument("--c3b", action="store_true",
+    ap.add_argument("--no-tricubic", action="store_true", help="skip the tricubic-variant measurement")
+    ap.add_argument("--no-tracers", action="store_true", help="skip the 64-tracer measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    ap.add_argument("--batch", action="store_true",
                     help="also measure the 256-problem persistent batch (about 200 GB of "
                          "device memory, a few minutes)")
-    ap.add_argument("--c3b-only", action="store_true", help="only the 256-problem batch")
-    ap.add_argument("--c3a-only", action="store_true", help="only the 64-tracer measurement")
-    ap.add_argument("--c4-only", action="store_true", help="only the adjoint measurement")
+    ap.add_argument("--batch-only", action="store_true", help="only the 256-problem batch")
+    ap.add_argument("--tracers-only", action="store_true", help="only the 64-tracer measurement")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
@@ -464,14 +464,14 @@ def main():
 
     import tbslas_amd as tb
 
-    if args.c3b_only:
-        print(json.dumps({"c3b_batch": c3b_measure(2)}))
+    if args.batch_only:
+        print(json.dumps({"batch_256_problems": batch_measure(2)}))
         return
-    if args.c3a_only:
-        print(json.dumps({"c3a_64_tracers": c3a_measure(2)}))
+    if args.tracers_only:
+        print(json.dumps({"tracers_64": tracers_measure(2)}))
         return
-    if args.c4_only:
-        print(json.dumps({"c4_adjoint": c4_measure()}))
+    if args.adjoint_only:
+        print(json.dumps({"adjoint_20_steps": adjoint_measure()}))
         return
     t_setup = time.perf_counter()
     anchor, depth, ccf, vcf = build_workload()
@@ -588,18 +588,18 @@ def main():
         "roofline_fp64": roof("fp64"),
     }
     out["locate"] = locate_measure(vel)
-    if not args.no_c5:
+    if not args.no_tricubic:
         del trees, vel
         torch.cuda.empty_cache()
-        out["c5_tricubic"] = c5_measure(10)
-    if not args.no_c3a:
+        out["tricubic_262144_leaves"] = tricubic_measure(10)
+    if not args.no_tracers:
         torch.cuda.empty_cache()
-        out["c3a_64_tracers"] = c3a_measure(2)
-    if not args.no_c4:
+        out["tracers_64"] = tracers_measure(2)
+    if not args.no_adjoint:
         torch.cuda.empty_cache()
-        out["c4_adjoint"] = c4_measure()
-    if args.c3b:
-        out["c3b_batch"] = c3b_measure(2)
+        out["adjoint_20_steps"] = adjoint_measure()
+    if args.batch:
+        out["batch_256_problems"] = batch_measure(2)
     if not args.no_cpu:
         v, npts, nl, t = cpu_baseline(anchor, depth, ccf, vcf, args.cpu_leaves)
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

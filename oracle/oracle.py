@@ -1,7 +1,7 @@
 """ctypes binding of the FP64 CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker / CPU baseline.  The engine
+cpu_baseline leg of bench.py (reference results / CPU baseline).  The engine
 (tbslas_amd, libtbx.so) never imports it.
 """
 from __future__ import annotations

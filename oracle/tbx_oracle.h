@@ -1,8 +1,8 @@
 /*
  * tbx oracle: FP64 CPU restatement of the tbslas semi-Lagrangian hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
- * engine (libtbx.so) is compared against, and the CPU baseline that the
+ * TEST INFRASTRUCTURE ONLY.  This library holds the reference results the
+ * HIP engine (libtbx.so) is compared against, and the CPU baseline that the
  * benchmark times beside it.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py load it.  The engine never links or calls it.
  *

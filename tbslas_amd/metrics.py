@@ -7,7 +7,7 @@ volume-weighted mean error (weights s^3 / n^3, s the leaf edge) is removed
 (utils.txx:158-187), and the absolute / relative L2 and L-inf errors of the
 remainder are reported (utils.txx:240-293).  The evaluation of the tree is
 a caller-supplied function, so the same metric serves device trees and
-the host checker.
+the host oracle.
 """
 from __future__ import annotations
 

@@ -5,7 +5,7 @@ reference), pinned by what the forward fixes:
   rounding (dot-product test);
 * the velocity cotangent agrees with central finite differences of the
   forward;
-* the 20-step C4 problem (uniform depth 5, q = 8, stationary random
+* the 20-step problem (uniform depth 5, q = 8, stationary random
   Chebyshev velocity, checkpoints every 5 steps): loss equal to the plain
   forward's, gradients equal to central differences (eps = 1e-6) along 32
   seeded directions (16 in c_0, 16 smooth velocity perturbations);
@@ -113,8 +113,8 @@ def _c4(tb, torch):
     return anchor, depth, q, vc, c0
 
 
-def test_c4_adjoint_against_finite_differences(torch, tb):
-    # C4: 20 steps of dt = 0.01 (|v| <= 1: at most a third of a leaf per
+def test_adjoint_20_steps_against_finite_differences(torch, tb):
+    # 20 steps of dt = 0.01 (|v| <= 1: at most a third of a leaf per
     # step), FreeSpace, target = the blob advected by the 10 % faster field
     import time
 
@@ -150,7 +150,7 @@ def test_c4_adjoint_against_finite_differences(torch, tb):
     plain = forward_loss(c0d, v0d)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
-    print(f"C4: loss {loss:.6e}, adjoint run {t_adj * 1e3:.1f} ms "
+    print(f"20-step adjoint: loss {loss:.6e}, adjoint run {t_adj * 1e3:.1f} ms "
           f"(forward {T} steps + backward), plain forward {t_fwd * 1e3:.1f} ms")
     assert abs(loss - plain) <= 1e-12 * plain
     # smooth velocity directions: seeded combinations of six fitted
@@ -188,7 +188,7 @@ def test_c4_adjoint_against_finite_differences(torch, tb):
         err = abs(fd - an) / abs(an)
         worst = max(worst, err)
         assert abs(fd - an) <= 1e-6 * abs(an) + 1e-10 * gn * float(d.norm()), (k, fd, an)
-    print(f"C4: worst relative FD mismatch over 32 directions {worst:.2e}")
+    print(f"20-step adjoint: worst relative FD mismatch over 32 directions {worst:.2e}")
 
 
 def test_autograd_function_matches_driver(torch, tb):

@@ -1,4 +1,4 @@
-"""Many tracers carried by one trajectory set (C3a at reduced size): a
+"""Many tracers carried by one trajectory set (the 64-tracer workload at a reduced size): a
 concentration tree with 64 components, one per tracer, each a seeded
 Gaussian blob (centres in [0.3, 0.7]^3, sigma in [0.03, 0.08], SplitMix64
 seed 3), advected by the rigid rotation of the CPU workload.
@@ -60,7 +60,7 @@ def test_64_tracers_share_one_trajectory_set(torch, tb, orc):
 
 
 def _small_problems(tb, n=12, seed=33):
-    # a reduced C3b: depths 2..3, degrees 6/8/10, rigid rotations about
+    # a reduced 256-problem batch: depths 2..3, degrees 6/8/10, rigid rotations about
     # random axes through the centre, a Gaussian blob each
     from tbslas_amd import fields
 

@@ -3,7 +3,7 @@ fast_interp (tree_functor.h:89-153) applied per leaf.
 
 Bars: TBX_EXACT bit-identical values, departure points and samples; TBX_FP64
 within 1e-13 relative; TBX_FP32 within 2e-6 relative (single-precision
-samples and sums).  At the full C5 size (262,144 leaves of 8^3 samples,
+samples and sums).  At the full size (262,144 leaves of 8^3 samples,
 134,217,728 departure points): a constant field stays constant and a linear
 field is advected exactly (both reproduced by the Lagrange stencil).
 """
@@ -75,8 +75,8 @@ def test_grid_step_against_oracle(torch, tb, orc, n, bc, timed):
         assert rel(g64.get_samples(), oc.samples) <= 1e-12, step
 
 
-def test_c5_full_size_step_properties(torch, tb):
-    # C5 shape: uniform depth 6, 8^3 samples per leaf, Lamb-Oseen pair
+def test_tricubic_full_size_step_properties(torch, tb):
+    # the large tricubic workload: uniform depth 6, 8^3 samples per leaf, Lamb-Oseen pair
     # (core 0.05, centres (0.4, 0.5), (0.6, 0.5)), dt = 5e-4, periodic.
     # The Lagrange stencil reproduces constants and linear functions, so a
     # constant stays constant and c = x + 2y - z moves with the departure

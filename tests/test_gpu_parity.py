@@ -223,7 +223,7 @@ def test_insitu_steps_against_oracle(torch, tb, orc, mode, bc, setup):
         assert err <= (FP64_TOL if mode != "fp32" else 1e-5), (step, err)
 
 
-def test_c1_full_size_step(torch, tb, orc):
+def test_uniform_depth4_full_size_step(torch, tb, orc):
     # the CPU-runnable workload at full size: uniform depth 4 (4096 leaves),
     # q = 8, v = 2 pi (0.5 - y, x - 0.5, 0), Gaussian blob sigma 0.06 at
     # (0.7, 0.5, 0.5), dt = 2 pi / 200; one step of 2,985,984 departure points
