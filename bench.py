@@ -172,15 +172,9 @@ def tricubic_measure(steps: int):
     return out
 
 
-def locate_measure(tree, n: int = 1 << 27, reps: int = 5):
-    """Bit-exact leaf location alone (tbx_locate: periodic wrap, 21-bit
-    keys, directory and search) of n uniformly random points on the
-    benchmark tree: HBM-bound, algorithmic bytes 24 (coordinates) + 4 (leaf
-    id) per point."""
+def _time_locate(tree, pts, reps: int):
     import torch
 
-    gen = torch.Generator(device="cuda").manual_seed(11)
-    pts = torch.rand((n, 3), dtype=torch.float64, device="cuda", generator=gen)
     tree.locate(pts, 1)  # warm-up
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -190,11 +184,36 @@ def locate_measure(tree, n: int = 1 << 27, reps: int = 5):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
+    n = pts.shape[0]
     nbytes = 28 * n
-    del pts
     return {"points": n, "ms": ms, "points_per_s": n / (ms * 1e-3),
             "algorithmic_bytes": nbytes, "hbm_gbs": nbytes / (ms * 1e-3) / 1e9,
             "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK}
+
+
+def locate_measure(tree, n: int = 1 << 27, reps: int = 5):
+    """Bit-exact leaf location alone (tbx_locate: periodic wrap, 21-bit
+    keys, directory and search) on the benchmark tree, HBM-bound with
+    algorithmic bytes 24 (coordinates) + 4 (leaf id) per point, for two
+    point sets: n uniformly random points (every directory read a random
+    one), and the departure points of the benchmark's first step (its grid
+    points traced back one dt), which arrive in leaf order as they do inside
+    a step."""
+    import torch
+
+    import tbslas_amd as tb
+
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    pts = torch.rand((n, 3), dtype=torch.float64, device="cuda", generator=gen)
+    out = {"uniform_random": _time_locate(tree, pts, reps)}
+    del pts
+    x = tree.grid_points()
+    dep = tb.traj_rk2(tree, x, DT, 0.0, 1, tb.BC_PERIODIC, "fp64", g=tfac)
+    del x
+    out["step_departure_points"] = _time_locate(tree, dep, reps)
+    del dep
+    torch.cuda.empty_cache()
+    return out
 
 
 def adjoint_measure():

@@ -11,42 +11,55 @@ namespace {
 // :190-198) without evaluation; pts are left untouched.  A block takes
 // kLocatePts x kBlock consecutive points: their coordinates (24 bytes a
 // point) come in through LDS with 16-byte loads, consecutive per lane, and
-// each thread then locates kLocatePts points a block apart.
+// each thread then locates kLocatePts points a block apart, all their
+// directory loads issued before the first search.  Coordinates and leaf ids
+// are streamed (non-temporal), which leaves the caches to the directory and
+// the keys.
 constexpr int kLocatePts = 4;
+typedef double stage2 __attribute__((ext_vector_type(2)));
 
 __global__ __launch_bounds__(kBlock) void k_locate(TreeView t,
                                                    const double *__restrict__ pts,
                                                    int64_t n, int bc,
                                                    int32_t *__restrict__ leaf) {
   constexpr int NP = kBlock * kLocatePts;  // points of the block
-  __shared__ double4 stage[NP * 3 / 4];    // NP x 3 doubles
+  __shared__ stage2 stage[NP * 3 / 2];     // NP x 3 doubles
   const int64_t p0 = (int64_t)blockIdx.x * NP;
   const int64_t np = n - p0 < NP ? n - p0 : NP;
   const int tid = threadIdx.x;
   if (np == NP && (reinterpret_cast<uintptr_t>(pts) & 15u) == 0) {
     // the block's 3 NP doubles as NP * 3 / 2 16-byte pieces
-    const double2 *src = reinterpret_cast<const double2 *>(pts + 3 * p0);
-    double2 *dst = reinterpret_cast<double2 *>(stage);
+    const stage2 *src = reinterpret_cast<const stage2 *>(pts + 3 * p0);
 #pragma unroll
-    for (int k = 0; k < NP * 3 / 2 / kBlock; ++k) dst[k * kBlock + tid] = src[k * kBlock + tid];
+    for (int k = 0; k < NP * 3 / 2 / kBlock; ++k)
+      stage[k * kBlock + tid] = __builtin_nontemporal_load(src + k * kBlock + tid);
   } else {
     double *dst = reinterpret_cast<double *>(stage);
     for (int k = tid; k < 3 * np; k += kBlock) dst[k] = pts[3 * p0 + k];
   }
   __syncthreads();
   const double *c = reinterpret_cast<const double *>(stage);
+  uint64_t key[kLocatePts];
+  int lo[kLocatePts], hi[kLocatePts];
+#pragma unroll
+  for (int k = 0; k < kLocatePts; ++k) {
+    // lanes past the end locate the block's first point and store nothing
+    const int j = k * kBlock + tid < np ? k * kBlock + tid : 0;
+    double a = c[3 * j + 0], b = c[3 * j + 1], z = c[3 * j + 2];
+    if (bc == TBX_BC_PERIODIC) {
+      a = wrap1(a);
+      b = wrap1(b);
+      z = wrap1(z);
+    }
+    key[k] = point_key(a, b, z);
+    lo[k] = dir_first(t, key[k]);
+    hi[k] = dir_last(t, key[k]);
+  }
 #pragma unroll
   for (int k = 0; k < kLocatePts; ++k) {
     const int j = k * kBlock + tid;
-    if (j < np) {
-      double a = c[3 * j + 0], b = c[3 * j + 1], z = c[3 * j + 2];
-      if (bc == TBX_BC_PERIODIC) {
-        a = wrap1(a);
-        b = wrap1(b);
-        z = wrap1(z);
-      }
-      leaf[p0 + j] = locate(t, a, b, z);
-    }
+    const int r = dir_search(t, key[k], lo[k], hi[k]);
+    if (j < np) __builtin_nontemporal_store(r, leaf + p0 + j);
   }
 }
 

@@ -82,11 +82,22 @@ __device__ __forceinline__ uint64_t point_key(double x, double y, double z) {
 
 // Index of the leaf whose key range holds key k: the last leaf with
 // key <= k.  The directory narrows the search to the leaves overlapping the
-// point's level-g cell; for trees no deeper than g it is a single lookup.
-__device__ __forceinline__ int locate_key(const TreeView &t, uint64_t k) {
-  const uint32_t c = (uint32_t)(k >> t.dir_shift);
-  int lo = t.dir[c];
-  int hi = t.dir[c + 1];
+// point's level-g cell, dir[c] .. dir[c + 1]; when no leaf is deeper than g
+// (dir_exact) the leaf holding the cell's first key holds the whole cell and
+// dir[c] is the answer.  dir_first / dir_search are the two halves, so that
+// a thread locating several points can issue all their directory loads
+// before it waits on any of them.
+__device__ __forceinline__ int dir_first(const TreeView &t, uint64_t k) {
+  return t.dir[(uint32_t)(k >> t.dir_shift)];
+}
+
+__device__ __forceinline__ int dir_last(const TreeView &t, uint64_t k) {
+  return t.dir_exact ? 0 : t.dir[(uint32_t)(k >> t.dir_shift) + 1];
+}
+
+__device__ __forceinline__ int dir_search(const TreeView &t, uint64_t k, int lo,
+                                          int hi) {
+  if (t.dir_exact) return lo;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (t.key[mid] <= k)
@@ -95,6 +106,10 @@ __device__ __forceinline__ int locate_key(const TreeView &t, uint64_t k) {
       hi = mid - 1;
   }
   return lo;
+}
+
+__device__ __forceinline__ int locate_key(const TreeView &t, uint64_t k) {
+  return dir_search(t, k, dir_first(t, k), dir_last(t, k));
 }
 
 __device__ __forceinline__ int locate(const TreeView &t, double x, double y,

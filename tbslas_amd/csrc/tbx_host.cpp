@@ -255,6 +255,7 @@ struct tbx_tree_s {
     v.nc = nc;
     v.q = q;
     v.dir_shift = 3 * (kKeyBits - dir_level);
+    v.dir_exact = max_depth <= dir_level ? 1 : 0;
     return v;
   }
   int64_t ncoeff_total() const { return L * dof * nc; }

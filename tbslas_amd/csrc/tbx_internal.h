@@ -33,6 +33,8 @@ struct TreeView {
   int32_t nc;
   int32_t q;
   int32_t dir_shift;     // 3 * (21 - g)
+  int32_t dir_exact;     // no leaf deeper than g: dir[c] is the leaf
+                         // that holds all of cell c
 };
 
 // A velocity functor of the kinds the reference's trajectory templates take.
