@@ -473,6 +473,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="only the 256-problem batch")
     ap.add_argument("--tracers-only", action="store_true", help="only the 64-tracer measurement")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement")
+    ap.add_argument("--locate-only", action="store_true",
+                    help="only the stand-alone locate measurement on the benchmark tree")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
@@ -500,6 +502,9 @@ def main():
     vel = tb.Tree(anchor, depth, Q, 3, vcf)
     print(f"# workload: {L} leaves, {N} departure points/step, setup "
           f"{time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+    if args.locate_only:
+        print(json.dumps({"locate": locate_measure(vel)}))
+        return
 
     from tbslas_amd import _lib
 

@@ -45,8 +45,10 @@ def rel(a, b):
 
 @pytest.mark.parametrize("seed", [1, 2, 3, 4])
 @pytest.mark.parametrize("bc", [0, 1])
-def test_locate_bit_exact(torch, tb, orc, seed, bc):
-    anchor, depth = random_tree(seed, max_depth=6, p_split=0.3)
+@pytest.mark.parametrize("max_depth", [6, 8])
+def test_locate_bit_exact(torch, tb, orc, seed, bc, max_depth):
+    # max depth 6: the directory names the leaf; 8: directory + search
+    anchor, depth = random_tree(seed, max_depth=max_depth, p_split=0.3)
     g, o = pair(tb, orc, anchor, depth, 2, 1, None)
     rng = np.random.default_rng(seed)
     pts = rng.random((40000, 3)) * (1.3 if bc else 1.0) - (0.15 if bc else 0.0)
