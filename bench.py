@@ -437,15 +437,16 @@ def batch_measure(steps: int = 2):
     return out
 
 
-def traffic_from_profiles(kernel: str):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
-    summary (profiles/step_traffic.txt: "kernel bytes" lines), if measured."""
+def traffic_from_profiles(kernel: str, shape: int):
+    """HBM bytes per launch of a kernel at a launch shape from the committed
+    rocprofv3 counter summary (profiles/step_traffic.txt: "kernel shape
+    bytes" lines), if measured."""
     p = os.path.join(ROOT, "profiles", "step_traffic.txt")
     try:
         for line in open(p):
             f = line.split()
-            if len(f) == 2 and not line.startswith("#") and f[0] == kernel:
-                return float(f[1])
+            if len(f) == 3 and not line.startswith("#") and f[0] == kernel and int(f[1]) == shape:
+                return float(f[2])
     except OSError:
         pass
     return None
@@ -458,7 +459,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-leaves", type=int, default=int(os.environ.get("TBX_CPU_LEAVES", 0)),
                     help="leaves in the CPU baseline step (0: all)")
-    ap.add_argument("--shape", type=int, default=None, help="step kernel launch shape")
+    ap.add_argument("--shape", type=int, default=None,
+                    help="step kernel launch shape (default -1: per arithmetic mode)")
     ap.add_argument("--profile", default=None, metavar="MODE",
                     help="counter-collection run: a 1 GiB device copy (the reference "
                          "for byte counters), then --steps step kernels of one "
@@ -510,7 +512,6 @@ def main():
 
     if args.shape is not None:
         _lib.set_option(_lib.OPT_STEP_SHAPE, args.shape)
-    shape = _lib.get_option(_lib.OPT_STEP_SHAPE)
 
     if args.profile:
         src = torch.full((1 << 27,), 1.0, dtype=torch.float64, device="cuda")
@@ -566,16 +567,17 @@ def main():
         del b
     del g, a
 
-    kname = {0: "k_step", 1: "k_step", 2: "k_step", 3: "k_step"}.get(shape, "k_step_rg")
     kmode = {"fp64": "kFma64", "fp32": "kFp32", "fp32h": "kFp32h"}
 
     def roof(mode):
         k_s = res[mode]["step_kernel_ms"] * 1e-3
         ach = flops_per_point(Q) * N / k_s / 1e12
         peak = PEAK["fp64" if mode == "fp64" else "fp32"]
+        shape = _lib.step_shape(mode, Q)
+        kname = "k_step" if shape in (0, 1, 2, 3) else "k_step_rg"
         name = f"{kname}<{kmode[mode]}>"
         return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": traffic_from_profiles(name),
+                "frac": ach / peak, "traffic": traffic_from_profiles(name, shape),
                 "kernel": name, "launch_shape": shape, "flops_per_point": flops_per_point(Q),
                 "points_per_launch": N}
 

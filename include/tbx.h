@@ -335,8 +335,12 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
  * step calls.  0: 256 threads; 1-3: 256 / 1024 / 512 threads with a
  * register budget of 4 waves per SIMD; 4-7: the workgroup's points
  * regrouped by leaf before each evaluation, 256 / 512 threads (budget 2
- * waves per SIMD) and 1024 / 512 threads (budget 4).  6 is the default;
- * shapes other than 0 and 6 exist for q = 8 and 14 only (others run 6).
+ * waves per SIMD) and 1024 / 512 threads (budget 4); 8-9: regrouped, 1024
+ * threads with a budget of 8 waves per SIMD (64 registers) and 512 threads
+ * with a budget of 6.  -1 (the default) picks per arithmetic mode and
+ * degree the shape measured fastest (q = 14: 8 for TBX_FP32, 9 for
+ * TBX_FP32_HEAD64; 6 otherwise); shapes other than 0 and 6 exist for
+ * q = 8 and 14 only (others run 6).
  * Every shape gives the same results.
  * TBX_OPT_TRACER_MFMA (default 1): a concentration with a multiple of 16
  * components (many tracers on one trajectory set) has its components
@@ -346,6 +350,11 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
 enum { TBX_OPT_STEP_SHAPE = 1, TBX_OPT_TRACER_MFMA = 2 };
 int tbx_set_option(int option, int value);
 int tbx_get_option(int option, int *value);
+
+/* The step launch shape (TBX_OPT_STEP_SHAPE) that a step with these mode
+ * flags at degree q runs with under the current option.  No counterpart in
+ * the reference: it reports the launch configuration for profiling. */
+int tbx_step_shape(uint32_t flags, int q, int *shape);
 
 const char *tbx_last_error(void);
 const char *tbx_version(void);

@@ -126,6 +126,7 @@ _SIGS = {
     "tbx_balance21": (_I32, [C.POINTER(_I64), _I64, _P, _P, _I32]),
     "tbx_set_option": (_I32, [_I32, _I32]),
     "tbx_get_option": (_I32, [_I32, C.POINTER(_I32)]),
+    "tbx_step_shape": (_I32, [C.c_uint32, _I32, C.POINTER(_I32)]),
     "tbx_last_error": (C.c_char_p, []),
     "tbx_version": (C.c_char_p, []),
 }
@@ -175,4 +176,11 @@ def set_option(option: int, value: int) -> None:
 def get_option(option: int) -> int:
     v = C.c_int()
     check(lib().tbx_get_option(option, C.byref(v)))
+    return v.value
+
+
+def step_shape(mode: str, q: int) -> int:
+    """The launch shape a step in `mode` at degree q runs with now."""
+    v = C.c_int()
+    check(lib().tbx_step_shape(MODES[mode], q, C.byref(v)))
     return v.value

@@ -27,7 +27,7 @@ using namespace tbx;
 namespace {
 
 thread_local std::string g_err;
-std::atomic<int> g_step_shape{kDefaultStepShape};
+std::atomic<int> g_step_shape{kAutoStepShape};
 std::atomic<int> g_tracer_mfma{1};
 
 int fail(int code, const char *fmt, ...) {
@@ -289,6 +289,14 @@ int ensure_f32(tbx_tree *t, hipStream_t s) {
     t->f32_stale = false;
   }
   return TBX_OK;
+}
+
+// The launch shape a step of this mode and degree runs with under the
+// current option (a shape not built for q falls back to the default).
+int resolve_step_shape(int mode, int q) {
+  int want = g_step_shape.load();
+  if (want == kAutoStepShape) want = auto_step_shape(mode, q);
+  return step_shape_built(want, q) ? want : kDefaultStepShape;
 }
 
 int mode_of(uint32_t flags, int *mode) {
@@ -779,8 +787,8 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
   if ((rc = prepare_mode(conc, mode, st))) return rc;
   const int P = conc->P();
   StepArgs a{};
-  const int want = g_step_shape.load();
-  a.shape = vs.kind == kVelTree && step_shape_built(want, conc->q) ? want : kDefaultStepShape;
+  const int want = resolve_step_shape(mode, conc->q);
+  a.shape = vs.kind == kVelTree ? want : kDefaultStepShape;
   // many components (tracers sharing the trajectories) on the matrix cores
   a.mfma = g_tracer_mfma.load() && vs.kind == kVelTree && (mode == kFma64 || mode == kFp32) &&
            conc->dof >= 16 && conc->dof % 16 == 0;
@@ -1513,8 +1521,8 @@ int tbx_batch_get_coeffs(const tbx_batch *b, int p, double *coeff, tbx_stream s)
 
 int tbx_set_option(int option, int value) {
   if (option == TBX_OPT_STEP_SHAPE) {
-    if (value < 0 || value >= kStepShapes)
-      return fail(TBX_EINVAL, "tbx_set_option: step shape %d outside 0..%d", value,
+    if (value < kAutoStepShape || value >= kStepShapes)
+      return fail(TBX_EINVAL, "tbx_set_option: step shape %d outside -1..%d", value,
                   kStepShapes - 1);
     g_step_shape = value;
     return TBX_OK;
@@ -1526,6 +1534,15 @@ int tbx_set_option(int option, int value) {
     return TBX_OK;
   }
   return fail(TBX_EINVAL, "tbx_set_option: unknown option %d", option);
+}
+
+int tbx_step_shape(uint32_t flags, int q, int *shape) {
+  if (!shape) return fail(TBX_EINVAL, "tbx_step_shape: null output");
+  if (q < 1 || q > TBX_MAX_Q) return fail(TBX_EINVAL, "tbx_step_shape: q %d outside 1..%d", q, TBX_MAX_Q);
+  int mode, rc;
+  if ((rc = mode_of(flags, &mode))) return rc;
+  *shape = resolve_step_shape(mode, q);
+  return TBX_OK;
 }
 
 int tbx_get_option(int option, int *value) {

@@ -134,6 +134,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
                     [&](int l, double v) { o[(int64_t)l * a.P] = v; });
 }
 
+// The start point of this thread's RK2 substep, read back from its row of
+// the regroup buffer after a velocity evaluation instead of being held in
+// registers through it (the row is this thread's until the next regroup,
+// whose first barrier every thread reaches after its evaluation).  The
+// empty asm with a memory clobber makes the compiler load it again rather
+// than keep the value it loaded at the regroup.
+template <int BLOCK, int NF>
+__device__ __forceinline__ void reload_start(RegroupLds<BLOCK, NF> &sh, double (&st)[6]) {
+  asm volatile("" ::: "memory");
+  st[0] = sh.f[0][threadIdx.x];
+  st[1] = sh.f[1][threadIdx.x];
+  st[2] = sh.f[2][threadIdx.x];
+}
+
 // The same step with the workgroup's points regrouped by leaf before each
 // of the 2 num_rk_step + 1 evaluations (regroup_pos in tbx_device.h).  The
 // thread that holds a point after a regroup carries it on: the RK2 state
@@ -180,6 +194,7 @@ __device__ __forceinline__ void k_step_rg_body(const VelSource &vel, const TreeV
       double v0, v1, v2;
       eval_velocity<D, MODE, KIND>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, st[0], st[1],
                              st[2], v0, v1, v2, lf);
+      reload_start(sh, st);
       // midpoint xtmp = xinit + (0.5 * dt) * vtmp (traj.inc:36), wrapped
       // for its evaluation only
       st[3] = st[0] + a.htau * v0;
@@ -194,6 +209,7 @@ __device__ __forceinline__ void k_step_rg_body(const VelSource &vel, const TreeV
       regroup(sh, orig, lf, st);
       eval_velocity<D, MODE, KIND>(vel, 2, a.ts[2 * s + 1], a.scaled, a.g[2 * s + 1], a.bc, st[3],
                              st[4], st[5], v0, v1, v2, lf);
+      reload_start(sh, st);
       // xsol = xinit + dt * vtmp (traj.inc:41-42)
       st[0] = st[0] + a.tau * v0;
       st[1] = st[1] + a.tau * v1;
@@ -341,6 +357,10 @@ hipError_t launch_step_mode(const VelSource &vel, const TreeView &conc, const St
       return launch_step_rg_shape<MODE, T, 512, 2>(vel, conc, a, vals, dep, s);
     case 7:
       return launch_step_rg_shape<MODE, T, 512, 4>(vel, conc, a, vals, dep, s);
+    case 8:
+      return launch_step_rg_shape<MODE, T, 1024, 8>(vel, conc, a, vals, dep, s);
+    case 9:
+      return launch_step_rg_shape<MODE, T, 512, 6>(vel, conc, a, vals, dep, s);
 #endif
     default:  // shape 6, and the shapes not built for this degree
       return launch_step_rg_shape<MODE, T, 1024, 4>(vel, conc, a, vals, dep, s);

@@ -363,8 +363,9 @@ def test_every_step_shape_matches_oracle(torch, tb, orc, q):
     gc = tb.Tree(anchor, depth, q, 2, cf)
     dep = torch.empty((gc.L * P, 3), dtype=torch.float64, device="cuda")
     old = _lib.get_option(_lib.OPT_STEP_SHAPE)
+    v32 = {m: tb.semilag_values(gv, gc, 2, dt, 2, 1, m).cpu().numpy() for m in ("fp32", "fp32h")}
     try:
-        for shape in range(8):
+        for shape in range(10):
             _lib.set_option(_lib.OPT_STEP_SHAPE, shape)
             dep.zero_()
             vals = tb.semilag_values(gv, gc, 2, dt, 2, 1, "exact", dep=dep)
@@ -372,6 +373,10 @@ def test_every_step_shape_matches_oracle(torch, tb, orc, q):
             assert np.array_equal(vals.cpu().numpy(), ref_val), shape
             v64 = tb.semilag_values(gv, gc, 2, dt, 2, 1, "fp64").cpu().numpy()
             assert rel(v64, ref_val) <= FP64_TOL, shape
+            # the FP32 modes: the same bits from every shape
+            for m in ("fp32", "fp32h"):
+                assert np.array_equal(tb.semilag_values(gv, gc, 2, dt, 2, 1, m).cpu().numpy(),
+                                      v32[m]), (shape, m)
     finally:
         _lib.set_option(_lib.OPT_STEP_SHAPE, old)
 

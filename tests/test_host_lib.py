@@ -169,3 +169,26 @@ def test_rotation_coefficients_closed_form():
         fit = tb.fit_function(lambda p: fields.rotation_axis(p, axis, 2.5), anchor, depth, q, 3)
         cf = fields.rotation_coefficients(anchor, depth, q, axis, 2.5)
         assert np.abs(fit - cf).max() <= 1e-13
+
+
+def test_step_shape_option(tb):
+    # -1 (the default) resolves per mode and degree; explicit shapes are
+    # taken where built for the degree and fall back to 6 elsewhere
+    from tbslas_amd import _lib
+    from tbslas_amd._lib import TbxError
+
+    old = _lib.get_option(_lib.OPT_STEP_SHAPE)
+    try:
+        _lib.set_option(_lib.OPT_STEP_SHAPE, -1)
+        assert _lib.get_option(_lib.OPT_STEP_SHAPE) == -1
+        assert [_lib.step_shape(m, 14) for m in ("fp64", "exact", "fp32", "fp32h")] == [6, 6, 8, 9]
+        assert [_lib.step_shape(m, 8) for m in ("fp64", "fp32", "fp32h")] == [6, 6, 6]
+        _lib.set_option(_lib.OPT_STEP_SHAPE, 7)
+        assert _lib.step_shape("fp32h", 14) == 7 and _lib.step_shape("fp32h", 5) == 6
+        for bad in (-2, 10):
+            with pytest.raises(TbxError):
+                _lib.set_option(_lib.OPT_STEP_SHAPE, bad)
+        with pytest.raises(TbxError):
+            _lib.step_shape("fp64", 0)
+    finally:
+        _lib.set_option(_lib.OPT_STEP_SHAPE, old)
