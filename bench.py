@@ -216,6 +216,44 @@ def locate_measure(tree, n: int = 1 << 27, reps: int = 5):
     return out
 
 
+def contraction_measure(tree, reps: int = 3):
+    """The leaf contraction alone (tbx_contraction_only) on the benchmark's
+    velocity tree: as many points as a step has departure points, 53
+    wavefronts per leaf (3,375 points, as a step's arrival points), the 3
+    components contracted one after another at each point.  Rate in
+    algorithmic flops, 2 x 815 per component (680 triangle terms + 135 for
+    the nesting at q = 14), against the peak of the mode's arithmetic."""
+    import torch
+
+    from tbslas_amd import _lib
+
+    n = tree.L * tree.P
+    wpl = -(-tree.P // 64)
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    lib = _lib.lib()
+    res = {"points": n, "waves_per_leaf": wpl, "components": tree.dof}
+    per_point = 2 * ((Q + 1) * (Q + 2) * (Q + 3) // 6 + (Q + 1) * (Q + 2) // 2 + (Q + 1)) * tree.dof
+    for mode in ("fp32", "fp32h", "fp64"):
+        flags = _lib.MODES[mode]
+        stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib.tbx_contraction_only(tree._h, n, wpl, flags, out.data_ptr(), None, stream))
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            _lib.check(lib.tbx_contraction_only(tree._h, n, wpl, flags, out.data_ptr(), None,
+                                                stream))
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tf = per_point * n / (ms * 1e-3) / 1e12
+        peak = PEAK["fp64" if mode == "fp64" else "fp32"]
+        res[mode] = {"ms": ms, "tflops": tf, "peak": peak, "frac": tf / peak}
+    res["flops_per_point"] = per_point
+    del out
+    return res
+
+
 def adjoint_measure():
     """The FP64 adjoint at full size: uniform depth 5 (32,768 leaves), q = 8,
     stationary random Chebyshev velocity (SplitMix64 seed 4, |v| <= 1), a
@@ -477,6 +515,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement")
     ap.add_argument("--locate-only", action="store_true",
                     help="only the stand-alone locate measurement on the benchmark tree")
+    ap.add_argument("--contraction-only", action="store_true",
+                    help="only the contraction-alone measurement on the benchmark tree")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
@@ -506,6 +546,9 @@ def main():
           f"{time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
     if args.locate_only:
         print(json.dumps({"locate": locate_measure(vel)}))
+        return
+    if args.contraction_only:
+        print(json.dumps({"contraction_only": contraction_measure(vel)}))
         return
 
     from tbslas_amd import _lib
@@ -614,6 +657,7 @@ def main():
         "roofline_fp64": roof("fp64"),
     }
     out["locate"] = locate_measure(vel)
+    out["contraction_only"] = contraction_measure(vel)
     if not args.no_tricubic:
         del trees, vel
         torch.cuda.empty_cache()

@@ -136,6 +136,15 @@ int tbx_locate(const tbx_tree *t, const double *pts, int64_t n, int bc,
 int tbx_eval(const tbx_tree *t, double *pts, int64_t n, int bc, uint32_t flags,
              double *out, tbx_stream s);
 
+/* ---- The leaf contraction alone (no counterpart in the reference: a
+ *      measurement of the evaluation's inner kernel without location, RK2
+ *      or regrouping).  Point i lies in leaf (i / 64 / waves_per_leaf) mod L
+ *      at a hashed position in its box; every component of that leaf is
+ *      contracted there in the mode of flags and out[i] = ((0 + u_0) + u_1)
+ *      + ... .  pts, if not null, receives the points ([n][3]). ---------- */
+int tbx_contraction_only(const tbx_tree *t, int64_t n, int waves_per_leaf, uint32_t flags,
+                         double *out, double *pts, tbx_stream s);
+
 /* ---- ComputeTrajRK2 / IntegrateRK2 (semilag/traj.inc:20-68) through the
  *      velocity tree's functor.  xsol may alias xinit (the reference runs in
  *      place, traj.inc:64).  Like the reference, the first stage of every

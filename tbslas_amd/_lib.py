@@ -127,6 +127,7 @@ _SIGS = {
     "tbx_set_option": (_I32, [_I32, _I32]),
     "tbx_get_option": (_I32, [_I32, C.POINTER(_I32)]),
     "tbx_step_shape": (_I32, [C.c_uint32, _I32, C.POINTER(_I32)]),
+    "tbx_contraction_only": (_I32, [_P, _I64, _I32, C.c_uint32, _P, _P, _P]),
     "tbx_last_error": (C.c_char_p, []),
     "tbx_version": (C.c_char_p, []),
 }

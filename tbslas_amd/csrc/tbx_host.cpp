@@ -334,6 +334,18 @@ hipError_t dispatch_eval(int q, int mode, const TreeView &t, double *pts,
   return hipErrorInvalidValue;
 }
 
+hipError_t dispatch_contract(int q, int mode, const TreeView &t, int64_t n, int wpl, double *out,
+                             double *pts, hipStream_t s) {
+  switch (q) {
+#define X(Q) \
+  case Q:    \
+    return launch_contract_q##Q(mode, t, n, wpl, out, pts, s);
+    TBX_Q_CASES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t dispatch_veval(int q, int mode, const VelSource &vel, double *pts, int64_t n,
                           int bc, int wb, double t, int scaled, double g, double *out,
                           hipStream_t s) {
@@ -677,6 +689,22 @@ int tbx_eval(const tbx_tree *tc, double *pts, int64_t n, int bc, uint32_t flags,
   if ((rc = prepare_mode(t, mode, (hipStream_t)s))) return rc;
   HIP_TRY(dispatch_eval(t->q, mode, t->view(), pts, n, bc,
                         (flags & TBX_WRITE_BACK_WRAP) ? 1 : 0, out, (hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_contraction_only(const tbx_tree *tc, int64_t n, int waves_per_leaf, uint32_t flags,
+                         double *out, double *pts, tbx_stream s) {
+  tbx_tree *t = const_cast<tbx_tree *>(tc);
+  if (!t) return fail(TBX_EINVAL, "tbx_contraction_only: null tree");
+  if (n < 0) return fail(TBX_EINVAL, "tbx_contraction_only: n = %lld", (long long)n);
+  if (waves_per_leaf < 1)
+    return fail(TBX_EINVAL, "tbx_contraction_only: waves_per_leaf %d < 1", waves_per_leaf);
+  if (n > 0 && !out) return fail(TBX_EINVAL, "tbx_contraction_only: null output");
+  if (n > (int64_t)INT32_MAX * kBlock) return fail(TBX_EOVERFLOW, "tbx_contraction_only: n too large");
+  int mode, rc;
+  if ((rc = mode_of(flags, &mode))) return rc;
+  if ((rc = prepare_mode(t, mode, (hipStream_t)s))) return rc;
+  HIP_TRY(dispatch_contract(t->q, mode, t->view(), n, waves_per_leaf, out, pts, (hipStream_t)s));
   return TBX_OK;
 }
 

@@ -134,7 +134,10 @@ constexpr int kAdjBlock = 256;  // workgroup of the adjoint concentration kernel
                                   double *lam, double *A, hipStream_t s);     \
   hipError_t launch_adj_vel_q##Q(int mode, const VelSource &vel,              \
                                  const TreeView &conc, const StepArgs &a,     \
-                                 const double *A, double *G, hipStream_t s);
+                                 const double *A, double *G, hipStream_t s);  \
+  hipError_t launch_contract_q##Q(int mode, const TreeView &t, int64_t n,     \
+                                  int waves_per_leaf, double *out,            \
+                                  double *pts, hipStream_t s);
 
 TBX_DECLARE_Q(1)
 TBX_DECLARE_Q(2)

@@ -286,6 +286,48 @@ hipError_t with_mode(int mode, F f) {
   }
 }
 
+// The leaf contraction alone, without the step around it: point i sits in
+// leaf (w / waves_per_leaf) mod L, w = i / 64 its wavefront, at a position in
+// the leaf's box from a hash of i; every component of the leaf is
+// contracted there one after another (as in a velocity evaluation) and
+// out[i] = ((0 + u_0) + u_1) + ... .  pts (optional) receives the points.
+// It measures the contraction's own rate, the ceiling of the step's.
+__device__ __forceinline__ double unit_hash(uint64_t v) {
+  v += 0x9E3779B97F4A7C15ull;  // SplitMix64 finaliser
+  v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+  v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+  v ^= v >> 31;
+  return (double)(v >> 11) * (1.0 / 9007199254740992.0);  // [0, 1)
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_contract_only(TreeView t, int64_t n,
+                                                          int waves_per_leaf,
+                                                          double *__restrict__ out,
+                                                          double *__restrict__ pts) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t w = __builtin_amdgcn_readfirstlane((int)((i >> 6) / waves_per_leaf % t.L));
+  const int lf = (int)w;
+  const uint4 inf = t.info[lf];
+  const double len = 1.0 / (double)(1u << inf.w);
+  const double x = (double)inf.x * len + len * unit_hash(3 * (uint64_t)i);
+  const double y = (double)inf.y * len + len * unit_hash(3 * (uint64_t)i + 1);
+  const double z = (double)inf.z * len + len * unit_hash(3 * (uint64_t)i + 2);
+  PointBasis<D, MODE> b;
+  b.build(t, x, y, z, lf);
+  double acc = 0.0;
+#pragma unroll 1
+  for (int c = 0; c < t.dof; ++c) acc += leaf_value<D, MODE>(t, lf, c, b);
+  if (i < n) {
+    out[i] = acc;
+    if (pts) {
+      pts[3 * i + 0] = x;
+      pts[3 * i + 1] = y;
+      pts[3 * i + 2] = z;
+    }
+  }
+}
+
 // Calls f with the velocity kind as a compile-time constant.
 template <typename F>
 hipError_t with_kind(int kind, F f) {
@@ -380,6 +422,17 @@ hipError_t TBX_CAT(launch_eval_q, TBX_Q)(int mode, const TreeView &t,
   return with_mode(mode, [&](auto m) {
     hipLaunchKernelGGL(k_eval<decltype(m)::value>, dim3(grid_for(n)), dim3(kBlock), 0, s, t, pts,
                        n, bc, write_back, out);
+    return hipGetLastError();
+  });
+}
+
+hipError_t TBX_CAT(launch_contract_q, TBX_Q)(int mode, const TreeView &t, int64_t n,
+                                             int waves_per_leaf, double *out, double *pts,
+                                             hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  return with_mode(mode, [&](auto m) {
+    hipLaunchKernelGGL(k_contract_only<decltype(m)::value>, dim3(grid_for(n)), dim3(kBlock), 0, s,
+                       t, n, waves_per_leaf, out, pts);
     return hipGetLastError();
   });
 }
