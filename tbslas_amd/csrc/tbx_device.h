@@ -355,27 +355,47 @@ __device__ __forceinline__ T load_uniform(const T *p) {
 
 // The head of one leaf's contraction in FP64: the terms with
 // i + j + k <= kHeadDeg, nested as the full contraction (k, then j, then i),
-// from the FP64 coefficients c (wave-uniform: scalar loads).
+// from the FP64 coefficients (wave-uniform: scalar loads).  load() reads
+// them before the FP32 contraction starts: the loads are volatile, so they
+// stay ahead of the contraction's own (volatile) coefficient stream and
+// their latency hides behind it instead of following it.
 template <int D>
-__device__ __forceinline__ double head_contract(const double *c, const double *hx,
-                                                const double *hy, const double *hz) {
-  constexpr int H = kHeadDeg < D - 1 ? kHeadDeg : D - 1;
-  double u = 0.0;
+struct Head {
+  static constexpr int H = kHeadDeg < D - 1 ? kHeadDeg : D - 1;
+  static constexpr int N = (H + 1) * (H + 2) * (H + 3) / 6;
+  double c[N];
+
+  __device__ __forceinline__ void load(const double *base) {
+    int n = 0;
 #pragma unroll
-  for (int i = 0; i <= H; ++i) {
-    double v = 0.0;
+    for (int i = 0; i <= H; ++i)
 #pragma unroll
-    for (int j = 0; i + j <= H; ++j) {
-      double w = 0.0;
+      for (int j = 0; i + j <= H; ++j)
 #pragma unroll
-      for (int k = 0; i + j + k <= H; ++k)
-        w = __builtin_fma(hx[k], load_uniform(c + tri_index(D, i, j, k)), w);
-      v = __builtin_fma(hy[j], w, v);
-    }
-    u = __builtin_fma(hz[i], v, u);
+        for (int k = 0; i + j + k <= H; ++k)
+          c[n++] = *(const volatile __attribute__((address_space(4))) double *)(
+              base + tri_index(D, i, j, k));
   }
-  return u;
-}
+
+  __device__ __forceinline__ double eval(const double *hx, const double *hy,
+                                         const double *hz) const {
+    double u = 0.0;
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i <= H; ++i) {
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; i + j <= H; ++j) {
+        double w = 0.0;
+#pragma unroll
+        for (int k = 0; i + j + k <= H; ++k) w = __builtin_fma(hx[k], c[n++], w);
+        v = __builtin_fma(hy[j], w, v);
+      }
+      u = __builtin_fma(hz[i], v, u);
+    }
+    return u;
+  }
+};
 
 // One component of one leaf at the lane's point, in the mode's arithmetic.
 template <int D, int MODE>
@@ -383,11 +403,14 @@ __device__ __forceinline__ double leaf_value(const TreeView &t, int lf, int comp
                                              const PointBasis<D, MODE> &b) {
   constexpr int NC = D * (D + 1) * (D + 2) / 6;
   const size_t off = ((size_t)lf * t.dof + comp) * NC;
-  const double u = (double)contract<D, MODE>(coeff_base<MODE>(t) + off, b.px, b.py, b.zx, b.z0);
-  if constexpr (MODE == kFp32h)
-    return u + head_contract<D>(t.coeff + off, b.hx, b.hy, b.hz);
-  else
-    return u;
+  if constexpr (MODE == kFp32h) {
+    Head<D> h;
+    h.load(t.coeff + off);
+    const double u = (double)contract<D, MODE>(coeff_base<MODE>(t) + off, b.px, b.py, b.zx, b.z0);
+    return u + h.eval(b.hx, b.hy, b.hz);
+  } else {
+    return (double)contract<D, MODE>(coeff_base<MODE>(t) + off, b.px, b.py, b.zx, b.z0);
+  }
 }
 
 // Velocity (dof 3) at one point: the waterfall over the wave's leaves.
