@@ -225,24 +225,18 @@ def contraction_measure(tree, reps: int = 3):
     the nesting at q = 14), against the peak of the mode's arithmetic."""
     import torch
 
-    from tbslas_amd import _lib
-
     n = tree.L * tree.P
     wpl = -(-tree.P // 64)
     out = torch.empty(n, dtype=torch.float64, device="cuda")
-    lib = _lib.lib()
     res = {"points": n, "waves_per_leaf": wpl, "components": tree.dof}
     per_point = 2 * ((Q + 1) * (Q + 2) * (Q + 3) // 6 + (Q + 1) * (Q + 2) // 2 + (Q + 1)) * tree.dof
     for mode in ("fp32", "fp32h", "fp64"):
-        flags = _lib.MODES[mode]
-        stream = torch.cuda.current_stream().cuda_stream
-        _lib.check(lib.tbx_contraction_only(tree._h, n, wpl, flags, out.data_ptr(), None, stream))
+        tree.contraction_only(n, wpl, mode, out=out)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            _lib.check(lib.tbx_contraction_only(tree._h, n, wpl, flags, out.data_ptr(), None,
-                                                stream))
+            tree.contraction_only(n, wpl, mode, out=out)
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps

@@ -174,6 +174,27 @@ class Tree:
         check(lib().tbx_eval(self._h, _ptr(pts), pts.shape[0], bc, flags, _ptr(out), _stream()))
         return out
 
+    def contraction_only(self, n: int, waves_per_leaf: int, mode: str = "fp64", out=None,
+                         pts=None):
+        """tbx_contraction_only: the leaf contraction alone at n hashed points,
+        point i in leaf (i // 64 // waves_per_leaf) mod L; out[i] is the sum
+        of the leaf's components there, pts (optional, [n][3]) receives the
+        points.  Returns out."""
+        import torch
+
+        if out is None:
+            out = torch.empty(n, dtype=torch.float64, device="cuda")
+        _require_dev(out, torch.float64, "out")
+        if out.shape[0] < n:
+            raise ValueError(f"out holds {out.shape[0]} values, need {n}")
+        if pts is not None:
+            _require_dev(pts, torch.float64, "pts", (3,))
+            if pts.shape[0] < n:
+                raise ValueError(f"pts holds {pts.shape[0]} points, need {n}")
+        check(lib().tbx_contraction_only(self._h, n, waves_per_leaf, _mode(mode), _ptr(out),
+                                         _ptr(pts) if pts is not None else None, _stream()))
+        return out
+
     def grid_points(self):
         import torch
 

@@ -416,19 +416,13 @@ def test_fp32_head64_mode(torch, tb, orc):
 def test_contraction_only_matches_oracle(torch, tb, orc, q, dof):
     # the stand-alone contraction: TBX_EXACT gives, at the points it reports,
     # the oracle's component values summed in order; FP64 within 1e-12
-    from tbslas_amd import _lib
-
     anchor, depth = random_tree(61 + q, max_depth=3)
     rng = np.random.default_rng(q)
     cf = rng.standard_normal((len(depth), dof, tb.ncoef(q)))
     g, o = tb.Tree(anchor, depth, q, dof, cf), orc.Tree(anchor, depth, q, dof, cf)
     n, wpl = 5000, 2
-    out = torch.empty(n, dtype=torch.float64, device="cuda")
     pts = torch.empty((n, 3), dtype=torch.float64, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
-    lib = _lib.lib()
-    _lib.check(lib.tbx_contraction_only(g._h, n, wpl, _lib.MODES["exact"], out.data_ptr(),
-                                        pts.data_ptr(), stream))
+    out = g.contraction_only(n, wpl, "exact", pts=pts)
     p = pts.cpu().numpy()
     # point i lies in leaf (i // 64 // wpl) mod L
     leaf = orc.locate(o, p, 0)
@@ -438,6 +432,5 @@ def test_contraction_only_matches_oracle(torch, tb, orc, q, dof):
     for c in range(dof):
         want = want + vals[:, c]
     assert np.array_equal(out.cpu().numpy(), want)
-    _lib.check(lib.tbx_contraction_only(g._h, n, wpl, _lib.MODES["fp64"], out.data_ptr(), None,
-                                        stream))
+    out = g.contraction_only(n, wpl, "fp64")
     assert rel(out.cpu().numpy(), want) <= 1e-12
