@@ -344,11 +344,11 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
  * step calls.  0: 256 threads; 1-3: 256 / 1024 / 512 threads with a
  * register budget of 4 waves per SIMD; 4-7: the workgroup's points
  * regrouped by leaf before each evaluation, 256 / 512 threads (budget 2
- * waves per SIMD) and 1024 / 512 threads (budget 4); 8-9: regrouped, 1024
- * threads with a budget of 8 waves per SIMD (64 registers) and 512 threads
- * with a budget of 6.  -1 (the default) picks per arithmetic mode and
- * degree the shape measured fastest (q = 14: 8 for TBX_FP32, 9 for
- * TBX_FP32_HEAD64; 6 otherwise); shapes other than 0 and 6 exist for
+ * waves per SIMD) and 1024 / 512 threads (budget 4); 8-10: regrouped,
+ * 1024 threads with a budget of 8 waves per SIMD (64 registers), 512 and
+ * 768 threads with a budget of 6.  -1 (the default) picks per arithmetic
+ * mode and degree the shape measured fastest (q = 14: 8 for TBX_FP32, 10
+ * for TBX_FP32_HEAD64; 6 otherwise); shapes other than 0 and 6 exist for
  * q = 8 and 14 only (others run 6).
  * Every shape gives the same results.
  * TBX_OPT_TRACER_MFMA (default 1): a concentration with a multiple of 16

@@ -72,20 +72,22 @@ struct SubArgs {
 // before every evaluation.  Shapes 0 and 6 (the default) are built for
 // every degree, the others for the benchmark degrees 8 and 14 only; an
 // unbuilt shape falls back to 6.
-constexpr int kStepShapes = 10;
-constexpr int kStepShapeBlock[kStepShapes] = {256, 256, 1024, 512, 256, 512, 1024, 512, 1024, 512};
+constexpr int kStepShapes = 11;
+constexpr int kStepShapeBlock[kStepShapes] = {256, 256, 1024, 512, 256, 512, 1024, 512, 1024, 512, 768};
 constexpr int kDefaultStepShape = 6;
 // The step shape option's default, -1: the shape measured fastest for the
 // step's arithmetic mode and degree.  At q = 14 (adaptive tree of 36,324
 // leaves, 122,593,500 points, MI355X) the FP32 step kernel runs in 33.6 ms
 // with shape 8 (1024 threads, 64 VGPRs: two workgroups per CU) against
-// 42.6 ms with shape 6, and FP32 + FP64 head in 40.9 ms with shape 9 (512
-// threads, 80 VGPRs, three per CU) against 44.9 ms; FP64 keeps shape 6
-// (73.3 ms against 98-102 ms, where the smaller budgets spill).
+// 42.6 ms with shape 6, and FP32 + FP64 head in 38.6 ms with shape 10 (768
+// threads, 80 VGPRs, two per CU) against 41.0 ms with shape 9 (512
+// threads, whose smaller regrouping leaves 1.49 leaf passes per wavefront)
+// and 44.9 ms with shape 6; FP64 keeps shape 6 (73.3 ms against 92-102 ms,
+// where the smaller budgets spill).
 constexpr int kAutoStepShape = -1;
 inline int auto_step_shape(int mode, int q) {
   if (q == 14 && mode == kFp32) return 8;
-  if (q == 14 && mode == kFp32h) return 9;
+  if (q == 14 && mode == kFp32h) return 10;
   return kDefaultStepShape;
 }
 inline bool step_shape_built(int shape, int q) {

@@ -403,6 +403,8 @@ hipError_t launch_step_mode(const VelSource &vel, const TreeView &conc, const St
       return launch_step_rg_shape<MODE, T, 1024, 8>(vel, conc, a, vals, dep, s);
     case 9:
       return launch_step_rg_shape<MODE, T, 512, 6>(vel, conc, a, vals, dep, s);
+    case 10:
+      return launch_step_rg_shape<MODE, T, 768, 6>(vel, conc, a, vals, dep, s);
 #endif
     default:  // shape 6, and the shapes not built for this degree
       return launch_step_rg_shape<MODE, T, 1024, 4>(vel, conc, a, vals, dep, s);

@@ -365,7 +365,7 @@ def test_every_step_shape_matches_oracle(torch, tb, orc, q):
     old = _lib.get_option(_lib.OPT_STEP_SHAPE)
     v32 = {m: tb.semilag_values(gv, gc, 2, dt, 2, 1, m).cpu().numpy() for m in ("fp32", "fp32h")}
     try:
-        for shape in range(10):
+        for shape in range(11):
             _lib.set_option(_lib.OPT_STEP_SHAPE, shape)
             dep.zero_()
             vals = tb.semilag_values(gv, gc, 2, dt, 2, 1, "exact", dep=dep)

@@ -181,11 +181,11 @@ def test_step_shape_option(tb):
     try:
         _lib.set_option(_lib.OPT_STEP_SHAPE, -1)
         assert _lib.get_option(_lib.OPT_STEP_SHAPE) == -1
-        assert [_lib.step_shape(m, 14) for m in ("fp64", "exact", "fp32", "fp32h")] == [6, 6, 8, 9]
+        assert [_lib.step_shape(m, 14) for m in ("fp64", "exact", "fp32", "fp32h")] == [6, 6, 8, 10]
         assert [_lib.step_shape(m, 8) for m in ("fp64", "fp32", "fp32h")] == [6, 6, 6]
         _lib.set_option(_lib.OPT_STEP_SHAPE, 7)
         assert _lib.step_shape("fp32h", 14) == 7 and _lib.step_shape("fp32h", 5) == 6
-        for bad in (-2, 10):
+        for bad in (-2, 11):
             with pytest.raises(TbxError):
                 _lib.set_option(_lib.OPT_STEP_SHAPE, bad)
         with pytest.raises(TbxError):
