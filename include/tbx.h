@@ -353,9 +353,9 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
  * Every shape gives the same results.
  * TBX_OPT_TRACER_MFMA (default 1): a concentration with a multiple of 16
  * components (many tracers on one trajectory set) has its components
- * contracted on the matrix cores in TBX_FP64 and TBX_FP32 steps (512-thread
- * regrouped workgroups, whatever the step shape); 0 keeps the vector
- * contraction.  The two agree to rounding (summation order). */
+ * contracted on the matrix cores in TBX_FP64 and TBX_FP32 steps (regrouped
+ * workgroups of 512 threads in FP64 and 256 in FP32, whatever the step
+ * shape); 0 keeps the vector contraction.  The two agree to rounding (summation order). */
 enum { TBX_OPT_STEP_SHAPE = 1, TBX_OPT_TRACER_MFMA = 2 };
 int tbx_set_option(int option, int value);
 int tbx_get_option(int option, int *value);

@@ -822,7 +822,8 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
            conc->dof >= 16 && conc->dof % 16 == 0;
   if (a.mfma) a.shape = kMfmaStepShape;
   a.npts = conc->L * P;
-  a.nblocks = (a.npts + kStepShapeBlock[a.shape] - 1) / kStepShapeBlock[a.shape];
+  const int block = a.mfma ? mfma_step_block(mode) : kStepShapeBlock[a.shape];
+  a.nblocks = (a.npts + block - 1) / block;
   a.per_xcd = (a.nblocks + 7) / 8;
   if (8 * a.per_xcd > INT32_MAX) return fail(TBX_EOVERFLOW, "%s: grid too large", who);
   a.P = P;

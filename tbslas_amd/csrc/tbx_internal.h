@@ -96,6 +96,11 @@ inline bool step_shape_built(int shape, int q) {
 // Workgroup size of the matrix-core step (many-component concentrations,
 // every degree): shape 5's 512 threads.
 constexpr int kMfmaStepShape = 5;
+// Workgroup of the matrix-core tracer step: FP64 at 512 threads (two waves
+// per SIMD: its tiles need 256 registers), FP32 at 256 threads with a
+// budget of three waves per SIMD (166 registers), three workgroups per CU.
+constexpr int mfma_step_block(int mode) { return mode == kFma64 ? 512 : 256; }
+constexpr int mfma_step_waves(int mode) { return mode == kFma64 ? 2 : 3; }
 
 struct StepArgs {
   int32_t shape;      // index into kStepShapeBlock

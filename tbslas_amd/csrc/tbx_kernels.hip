@@ -362,13 +362,14 @@ hipError_t launch_step_rg_shape(const VelSource &vel, const TreeView &conc, cons
 template <int MODE>
 hipError_t launch_step_mode(const VelSource &vel, const TreeView &conc, const StepArgs &a,
                             double *vals, double *dep, hipStream_t s) {
-  // many components on the matrix cores: 512-thread workgroups, two waves
-  // per SIMD (the tile operands and accumulators need the registers)
+  // many components on the matrix cores: workgroups of mfma_step_block
+  // threads (the tile operands and accumulators need the registers)
   if (a.mfma) {
     if constexpr (MODE == kFma64 || MODE == kFp32) {
       if (vel.kind != kVelTree || a.shape != kMfmaStepShape) return hipErrorInvalidValue;
-      hipLaunchKernelGGL((k_step_rg_mfma<MODE, kVelTree, 512, 2>), dim3((unsigned)(8 * a.per_xcd)),
-                         dim3(512), 0, s, vel, conc, a, vals, dep);
+      constexpr int B = mfma_step_block(MODE), W = mfma_step_waves(MODE);
+      hipLaunchKernelGGL((k_step_rg_mfma<MODE, kVelTree, B, W>), dim3((unsigned)(8 * a.per_xcd)),
+                         dim3(B), 0, s, vel, conc, a, vals, dep);
       return hipGetLastError();
     } else {
       return hipErrorInvalidValue;
