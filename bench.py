@@ -35,11 +35,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "departure-pt interps/s on 1 GPU: batched FP32 vs plain FP64 at L∞≤1e-6"
-Q = 14
-SEED_DEPTH, MAX_DEPTH, TOL = 5, 7, 1e-13
-SIGMA, CENTRE = 0.04, (0.7, 0.5, 0.5)
-DT = 1e-3
+from tbslas_amd import workloads as W  # noqa: E402
+
+Q = W.ADAPTIVE_Q
+DT = W.ADAPTIVE_DT
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X published dense
+# This build issues no packed FP32 instructions, so an FP32 vector kernel
+# issues one FMA per lane per clock, the FP64 rate (64 FMA/clock/CU).
+PEAK_FP32_UNPACKED = 78.6
 MFMA_PEAK_F64, MFMA_PEAK_F32 = PEAK["fp64"], PEAK["fp32"]  # dense matrix cores
 HBM_PEAK = 8000.0  # GB/s
 
@@ -55,19 +58,10 @@ def flops_per_point(q: int) -> int:
 
 
 def build_workload():
-    import tbslas_amd as tb
-    from tbslas_amd import fields
-
-    fn = lambda p: fields.gaussian(p, CENTRE, SIGMA)
-    anchor, depth = tb.adaptive_leaves(fn, Q, 1, TOL, max_depth=MAX_DEPTH, min_depth=SEED_DEPTH,
-                                       bc=tb.BC_PERIODIC)
-    ccf = tb.fit_function(fn, anchor, depth, Q, 1)
-    vcf = tb.fit_function(fields.taylor_green_2d3, anchor, depth, Q, 3)
-    return anchor, depth, ccf, vcf
+    return W.adaptive_q14()
 
 
-def tfac(t: float) -> float:
-    return math.cos(math.pi * t)
+tfac = W.taylor_green_tfac
 
 
 def run_mode(mode, vel, anchor, depth, ccf, steps, warmup):
@@ -98,36 +92,148 @@ def run_mode(mode, vel, anchor, depth, ccf, steps, warmup):
     return conc, wall, k_ms, r_ms
 
 
-def cpu_baseline(anchor, depth, ccf, vcf, n_leaves: int):
-    """FP64 oracle (the reference algorithm restated, OpenMP on the host's
-    cores) for one SolveSemilagInSitu step on n_leaves leaves of the same
-    tree (all of them by default): arrival points, RK2 through the velocity
-    tree, concentration at the departure points, refit."""
+def host_cpu() -> dict:
+    """CPU model and core counts of this host (lscpu), and the CPUs this
+    process may run on."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                             text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+
+    def num(k):
+        try:
+            return int(info.get(k, ""))
+        except ValueError:
+            return None
+
+    cps, sockets = num("Core(s) per socket"), num("Socket(s)")
+    return {"model": info.get("Model name"),
+            "physical_cores": cps * sockets if cps and sockets else None,
+            "threads_per_core": num("Thread(s) per core"),
+            "cpus_allowed": len(os.sched_getaffinity(0))}
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
+    return orc
+
+
+def cpu_baseline(anchor, depth, ccf, vcf, frac: float, reps: int = 5):
+    """The FP64 oracle (the reference algorithm restated, OpenMP on the
+    host's cores) timed on a sample of the benchmark step: the arrival points
+    of a seeded random `frac` of the leaves, traced back by RK2 through the
+    whole velocity tree, the concentration evaluated at their departure
+    points on the whole tree, and those leaves refit.  One warm-up step,
+    then the median of `reps` timed steps."""
+    orc = _oracle()
     L = len(depth)
-    n_leaves = min(n_leaves, L) if n_leaves > 0 else L
+    n_leaves = max(1, int(round(frac * L)))
+    sel = np.sort(np.random.default_rng(0).choice(L, n_leaves, replace=False))
     ov = orc.Tree(anchor, depth, Q, 3, vcf)
-    if n_leaves < L:
-        sel = np.sort(np.random.default_rng(0).choice(L, n_leaves, replace=False))
-        oc = orc.Tree(anchor, depth, Q, 1, ccf)
-        sub = orc.Tree(anchor[sel], depth[sel], Q, 1, ccf[sel])
-        pts = orc.collect_grid_points(sub)
-        M = orc.pt2coeff(Q)
-        t0 = time.perf_counter()
+    oc = orc.Tree(anchor, depth, Q, 1, ccf)
+    sub = orc.Tree(anchor[sel], depth[sel], Q, 1, ccf[sel])
+    pts = orc.collect_grid_points(sub)
+    M = orc.pt2coeff(Q)
+
+    def step():
         vals = orc.semilag_rk2(ov, oc, pts, 1, DT, 1, orc.BC_PERIODIC, g=tfac)
         orc.set_grid_values(sub, M, vals.reshape(n_leaves, 1, -1))
-        t = time.perf_counter() - t0
-        npts = len(pts)
-    else:
-        oc = orc.Tree(anchor, depth, Q, 1, ccf)
-        M = orc.pt2coeff(Q)
+
+    step()
+    times = []
+    for _ in range(reps):
         t0 = time.perf_counter()
-        orc.semilag_insitu(ov, oc, M, 1, DT, 1, orc.BC_PERIODIC, g=tfac)
+        step()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"interps_per_s": 3 * len(pts) / t, "points": len(pts), "leaves": n_leaves,
+            "median_s": t, "times_s": times}
+
+
+def oracle_step_reference(anchor, depth, ccf, vcf):
+    """One full oracle step of the benchmark workload (timestep 1): the
+    concentration coefficients every GPU mode is compared with."""
+    orc = _oracle()
+    ov = orc.Tree(anchor, depth, Q, 3, vcf)
+    oc = orc.Tree(anchor, depth, Q, 1, ccf)
+    t0 = time.perf_counter()
+    orc.semilag_insitu(ov, oc, orc.pt2coeff(Q), 1, DT, 1, orc.BC_PERIODIC, g=tfac)
+    return oc.coeff.copy(), time.perf_counter() - t0
+
+
+def linf_rel(tree, coeff, pts, bc):
+    """max |u - u_ref| / max |u_ref| over pts, u the tree's values and u_ref
+    those of the same leaves holding coeff, both evaluated in TBX_EXACT."""
+    import tbslas_amd as tb
+
+    ref = tb.Tree(tree.anchor, tree.depth, tree.q, tree.dof)
+    ref.set_coeffs(coeff)
+    a = tree.eval(pts.clone(), bc, "exact")
+    b = ref.eval(pts.clone(), bc, "exact")
+    del ref
+    return float((a - b).abs().max() / b.abs().max())
+
+
+def uniform_q8_measure(cpu: bool):
+    """The uniform depth-4, degree-8 case (4,096 leaves, 2,985,984 points per
+    step, rigid rotation, Gaussian blob, free space, dt = 2 pi / 200) run in
+    full: 50 in-situ steps per GPU mode, and the same 50 steps of the FP64
+    oracle (OpenMP on the host's cores); each GPU mode's relative L-inf
+    against the oracle after the 50 steps at every grid point."""
+    import torch
+
+    import tbslas_amd as tb
+
+    anchor, depth, ccf, vcf = W.uniform_q8()
+    q, dt, bc, T = W.UNIFORM_Q, W.UNIFORM_DT, W.UNIFORM_BC, W.UNIFORM_STEPS
+    vel = tb.Tree(anchor, depth, q, 3, vcf)
+    N = len(depth) * (q + 1) ** 3
+    out = {"leaves": len(depth), "q": q, "departure_points_per_step": N, "steps": T, "dt": dt,
+           "bc": "free space"}
+    trees = {}
+    for mode in ("fp64", "fp32", "fp32h"):
+        conc = tb.Tree(anchor, depth, q, 1, ccf)
+        tb.semilag_insitu(vel, conc, 1, dt, 1, bc, mode)  # warm-up
+        conc.set_coeffs(ccf)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(1, T + 1):
+            tb.semilag_insitu(vel, conc, k, dt, 1, bc, mode)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / T
+        out[mode] = {"ms_per_step": ms, "interps_per_s": 3 * N / (ms * 1e-3),
+                     "departure_points_per_s": N / (ms * 1e-3)}
+        trees[mode] = conc
+    if cpu:
+        orc = _oracle()
+        ov, oc = orc.Tree(anchor, depth, q, 3, vcf), orc.Tree(anchor, depth, q, 1, ccf)
+        M = orc.pt2coeff(q)
+        t0 = time.perf_counter()
+        for k in range(1, T + 1):
+            orc.semilag_insitu(ov, oc, M, k, dt, 1, bc)
         t = time.perf_counter() - t0
-        npts = L * (Q + 1) ** 3
-    return 3 * npts / t, npts, n_leaves, t
+        out["cpu_oracle"] = {"s_total": t, "ms_per_step": 1e3 * t / T,
+                             "interps_per_s": 3 * N * T / t,
+                             "threads": int(os.environ.get("OMP_NUM_THREADS", "0")) or None}
+        pts = trees["fp64"].grid_points()
+        out["linf_rel_vs_oracle_after_all_steps"] = {
+            m: linf_rel(trees[m], oc.coeff, pts, bc) for m in trees}
+        out["fp64_over_cpu_oracle"] = out["fp64"]["interps_per_s"] / out["cpu_oracle"][
+            "interps_per_s"]
+        del pts
+    del trees, vel
+    return out
 
 
 def tricubic_measure(steps: int):
@@ -402,27 +508,13 @@ def tracers_measure(steps: int = 2):
     return out
 
 
-def batch_problems(n: int = 256, seed: int = 33):
-    """The batch workload: n problems with depth uniform in {3, 4, 5, 6},
-    degree in {6, 8, 10}, a rigid rotation (omega = 2 pi) about a uniformly
-    random axis through the centre, a Gaussian blob (sigma 0.08 at (0.6,
-    0.5, 0.5)) and dt = 1/200, drawn from SplitMix64(seed) as (z, phi,
-    depth, degree) per problem."""
-    from tbslas_amd import fields
-
-    u = fields.splitmix64(seed, 4 * n).reshape(n, 4)
-    out = []
-    for p in range(n):
-        z, phi = 2 * u[p, 0] - 1, 2 * math.pi * u[p, 1]
-        r = math.sqrt(max(0.0, 1 - z * z))
-        axis = (r * math.cos(phi), r * math.sin(phi), z)
-        out.append((3 + int(4 * u[p, 2]), (6, 8, 10)[int(3 * u[p, 3])], axis))
-    return out
-
-
-def batch_measure(steps: int = 2):
-    """256 small problems (batch_problems) in one persistent graph-captured
-    launch, FP32 contraction; the launch configuration autotuned."""
+def batch_measure(steps: int = 2, nprob: int = 256):
+    """The first nprob of the 256 small problems (workloads.batch_problems:
+    depth in {3..6}, degree in {6, 8, 10}, rigid rotation about a random
+    axis, a Gaussian blob, dt = 1/200) in one persistent graph-captured
+    launch, FP32 contraction; the launch configuration autotuned.  Also the
+    FP32 state after the steps against per-problem FP64 steps on the device
+    (relative L-inf of the coefficients, worst problem)."""
     import torch
 
     import tbslas_amd as tb
@@ -430,19 +522,18 @@ def batch_measure(steps: int = 2):
     from tbslas_amd.batch import CONFIGS, Batch
 
     t0 = time.perf_counter()
-    vel, conc, dts = [], [], []
+    vel, conc, dts, init = [], [], [], []
     npts = 0
-    for depth_u, q, axis in batch_problems():
+    for depth_u, q, axis in W.batch_problems()[:nprob]:
         anchor, depth = tb.uniform_leaves(depth_u)
         vel.append(tb.Tree(anchor, depth, q, 3,
                            fields.rotation_coefficients(anchor, depth, q, axis, 2 * math.pi)))
         c = tb.Tree(anchor, depth, q, 1)
         gp = c.grid_points()
-        v = torch.exp(-((gp - torch.tensor([0.6, 0.5, 0.5], dtype=torch.float64,
-                                           device="cuda")) ** 2).sum(1) / (2 * 0.08 ** 2))
-        c.set_grid_values(v)
-        del gp, v
+        c.set_grid_values(W.gaussian_grid_values(gp))
+        del gp
         conc.append(c)
+        init.append(c.get_coeffs_device())
         dts.append(1.0 / 200)
         npts += len(depth) * (q + 1) ** 3
     torch.cuda.empty_cache()
@@ -456,15 +547,27 @@ def batch_measure(steps: int = 2):
     e1.record()
     torch.cuda.synchronize()
     step_ms = e0.elapsed_time(e1) / steps
-    out = {"problems": len(vel), "departure_points_per_step": npts,
+    # tuning repeats one step of the current parity (same input buffer), so
+    # the state has advanced by the timed steps only
+    taken = steps
+    worst = 0.0
+    for p in range(0, len(vel), max(1, len(vel) // 16)):
+        ref = tb.Tree(conc[p].anchor, conc[p].depth, conc[p].q, 1, init[p])
+        for _ in range(taken):
+            tb.semilag_insitu(vel[p], ref, 1, dts[p], 1, 0, "fp64")
+        r = ref.get_coeffs()
+        worst = max(worst, float(np.abs(b.coeffs(p) - r).max() / np.abs(r).max()))
+        del ref
+    out = {"problems": len(vel), "of": 256, "departure_points_per_step": npts,
            "interps_per_step": 3 * npts, "setup_s": setup,
            "tuned_config": {"threads": CONFIGS[b.config][0],
                             "leaves_per_item": CONFIGS[b.config][1]},
            "tune_ms": {f"{t}x{lpi}": float(m) for (t, lpi), m in zip(CONFIGS, ms)},
            "ms_per_step": step_ms, "interps_per_s": 3 * npts / (step_ms * 1e-3),
-           "departure_points_per_s": npts / (step_ms * 1e-3), "dtype": "f32"}
+           "departure_points_per_s": npts / (step_ms * 1e-3), "dtype": "f32",
+           "coeff_linf_rel_vs_fp64_steps": worst, "steps_compared": taken}
     b.close()
-    del vel, conc
+    del vel, conc, init
     torch.cuda.empty_cache()
     return out
 
@@ -489,24 +592,29 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-leaves", type=int, default=int(os.environ.get("TBX_CPU_LEAVES", 0)),
-                    help="leaves in the CPU baseline step (0: all)")
+    ap.add_argument("--cpu-frac", type=float, default=float(os.environ.get("TBX_CPU_FRAC", 0.125)),
+                    help="fraction of the leaves whose arrival points the CPU baseline step "
+                         "traces (a seeded random sample)")
     ap.add_argument("--shape", type=int, default=None,
                     help="step kernel launch shape (default -1: per arithmetic mode)")
     ap.add_argument("--profile", default=None, metavar="MODE",
                     help="counter-collection run: a 1 GiB device copy (the reference "
                          "for byte counters), then --steps step kernels of one "
                          "arithmetic mode (fp64, fp32, fp32h, exact); no JSON line")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip every host oracle run (CPU baseline and oracle L-inf)")
     ap.add_argument("--no-tricubic", action="store_true", help="skip the tricubic-variant measurement")
     ap.add_argument("--no-tracers", action="store_true", help="skip the 64-tracer measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
-    ap.add_argument("--batch", action="store_true",
-                    help="also measure the 256-problem persistent batch (about 200 GB of "
-                         "device memory, a few minutes)")
+    ap.add_argument("--no-uniform", action="store_true", help="skip the uniform degree-8 case")
+    ap.add_argument("--batch-problems", type=int, default=32,
+                    help="problems of the persistent batch measured in the default run "
+                         "(0: none; the first n of the 256)")
     ap.add_argument("--batch-only", action="store_true", help="only the 256-problem batch")
     ap.add_argument("--tracers-only", action="store_true", help="only the 64-tracer measurement")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement")
+    ap.add_argument("--tricubic-only", action="store_true", help="only the tricubic measurement")
+    ap.add_argument("--uniform-only", action="store_true", help="only the uniform degree-8 case")
     ap.add_argument("--locate-only", action="store_true",
                     help="only the stand-alone locate measurement on the benchmark tree")
     ap.add_argument("--contraction-only", action="store_true",
@@ -516,19 +624,29 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
               file=sys.stderr)
         sys.exit(2)
+    # the CPU baseline's thread placement (set before the oracle's OpenMP
+    # runtime starts)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
 
     import torch
 
     import tbslas_amd as tb
 
     if args.batch_only:
-        print(json.dumps({"batch_256_problems": batch_measure(2)}))
+        print(json.dumps({"batch_256_problems": batch_measure(2, 256)}))
         return
     if args.tracers_only:
         print(json.dumps({"tracers_64": tracers_measure(2)}))
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint_20_steps": adjoint_measure()}))
+        return
+    if args.tricubic_only:
+        print(json.dumps({"tricubic_262144_leaves": tricubic_measure(10)}))
+        return
+    if args.uniform_only:
+        print(json.dumps({"uniform_4096_leaves_q8": uniform_q8_measure(not args.no_cpu)}))
         return
     t_setup = time.perf_counter()
     anchor, depth, ccf, vcf = build_workload()
@@ -562,7 +680,11 @@ def main():
             print(f"# {args.profile} step {s} done", file=sys.stderr, flush=True)
         return
 
-    # L-inf of each FP32 mode against FP64 after one step, at every grid point
+    # every mode after one step, against FP64 on the device and (unless
+    # --no-cpu) against one full oracle step, at every grid point
+    if not args.no_cpu:
+        ref1, t_ref = oracle_step_reference(anchor, depth, ccf, vcf)
+        print(f"# oracle step: {t_ref:.1f} s", file=sys.stderr, flush=True)
     first = {}
     g = None
     for mode in ("fp64", "fp32", "fp32h"):
@@ -570,10 +692,13 @@ def main():
         tb.semilag_insitu(vel, c1, 1, DT, 1, tb.BC_PERIODIC, mode, g=tfac)
         if g is None:
             g = c1.grid_points()
-        first[mode] = c1.eval(g.clone(), tb.BC_PERIODIC, "fp64")
-        del c1
-    linf1 = {m: float((first[m] - first["fp64"]).abs().max() / first["fp64"].abs().max())
+        first[mode] = c1
+    linf1 = {m: linf_rel(first[m], first["fp64"].get_coeffs_device(), g, tb.BC_PERIODIC)
              for m in ("fp32", "fp32h")}
+    linf1_orc = None
+    if not args.no_cpu:
+        linf1_orc = {m: linf_rel(first[m], ref1, g, tb.BC_PERIODIC) for m in first}
+        del ref1
     del first
 
     res = {}
@@ -595,14 +720,11 @@ def main():
         print(f"# {mode}: {ms:.2f} ms/step (step kernel {k_ms:.2f} ms, refit {r_ms:.2f} ms)",
               file=sys.stderr, flush=True)
 
-    # L-inf of each FP32 mode against FP64 after the timed steps
-    a = trees["fp64"].eval(g.clone(), tb.BC_PERIODIC, "fp64")
-    linf = {}
-    for m in ("fp32", "fp32h"):
-        b = trees[m].eval(g.clone(), tb.BC_PERIODIC, "fp64")
-        linf[m] = float((a - b).abs().max() / a.abs().max())
-        del b
-    del g, a
+    # each FP32 mode against FP64 after the timed steps (the FP64 path is
+    # pinned to the oracle by the one-step figures above)
+    c64 = trees["fp64"].get_coeffs_device()
+    linf = {m: linf_rel(trees[m], c64, g, tb.BC_PERIODIC) for m in ("fp32", "fp32h")}
+    del g, c64
 
     kmode = {"fp64": "kFma64", "fp32": "kFp32", "fp32h": "kFp32h"}
 
@@ -613,10 +735,17 @@ def main():
         shape = _lib.step_shape(mode, Q)
         kname = "k_step" if shape in (0, 1, 2, 3) else "k_step_rg"
         name = f"{kname}<{kmode[mode]}>"
-        return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": traffic_from_profiles(name, shape),
-                "kernel": name, "launch_shape": shape, "flops_per_point": flops_per_point(Q),
-                "points_per_launch": N}
+        r = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+             "frac": ach / peak, "traffic": traffic_from_profiles(name, shape),
+             "traffic_source": "profiles/step_traffic.txt (rocprofv3 --pmc FETCH_SIZE and "
+                               "WRITE_SIZE of this kernel at this shape, scaled by a 1 GiB copy)",
+             "kernel": name, "launch_shape": shape, "flops_per_point": flops_per_point(Q),
+             "points_per_launch": N, "timing": "HIP events on the launch stream"}
+        if mode != "fp64":
+            r["peak_note"] = ("published FP32 vector peak, reached only by packed FP32, which "
+                              "this build does not issue")
+            r["frac_of_unpacked_fp32"] = ach / PEAK_FP32_UNPACKED
+        return r
 
     out = {
         "metric": METRIC,
@@ -644,34 +773,47 @@ def main():
         "fp32h": res["fp32h"],
         "fp32h_over_fp64": res["fp32h"]["value"] / res["fp64"]["value"],
         "fp32_over_fp64": res["fp32"]["value"] / res["fp64"]["value"],
+        "linf_rel_vs_oracle_after_1_step": linf1_orc,
         "linf_rel_vs_fp64_after_1_step": linf1,
         "linf_rel_vs_fp64_after_all_steps": linf,
         "roofline": roof("fp32h"),
         "roofline_fp32": roof("fp32"),
         "roofline_fp64": roof("fp64"),
     }
+    del trees
     out["locate"] = locate_measure(vel)
     out["contraction_only"] = contraction_measure(vel)
+    del vel
+    torch.cuda.empty_cache()
+    if not args.no_uniform:
+        out["uniform_4096_leaves_q8"] = uniform_q8_measure(not args.no_cpu)
+        torch.cuda.empty_cache()
     if not args.no_tricubic:
-        del trees, vel
-        torch.cuda.empty_cache()
         out["tricubic_262144_leaves"] = tricubic_measure(10)
+        torch.cuda.empty_cache()
     if not args.no_tracers:
-        torch.cuda.empty_cache()
         out["tracers_64"] = tracers_measure(2)
-    if not args.no_adjoint:
         torch.cuda.empty_cache()
+    if not args.no_adjoint:
         out["adjoint_20_steps"] = adjoint_measure()
-    if args.batch:
-        out["batch_256_problems"] = batch_measure(2)
+        torch.cuda.empty_cache()
+    if args.batch_problems > 0:
+        out["batch_problems"] = batch_measure(2, min(256, args.batch_problems))
+        torch.cuda.empty_cache()
     if not args.no_cpu:
-        v, npts, nl, t = cpu_baseline(anchor, depth, ccf, vcf, args.cpu_leaves)
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        what = "all" if nl == L else f"{nl} random"
+        cb = cpu_baseline(anchor, depth, ccf, vcf, args.cpu_frac)
+        cpu = host_cpu()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or cpu["cpus_allowed"]
         out["cpu_baseline"] = {
-            "value": v, "unit": "interps/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle (OpenMP), one step (RK2 + concentration + refit) on the "
-                      f"{npts} arrival points of {what} leaves of the same tree ({t:.1f} s)",
+            "value": cb["interps_per_s"], "unit": "interps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (OpenMP, {threads} threads, OMP_PROC_BIND="
+                      f"{os.environ.get('OMP_PROC_BIND')}, OMP_PLACES="
+                      f"{os.environ.get('OMP_PLACES')}): one in-situ step of the benchmark "
+                      f"workload restricted to the {cb['points']} arrival points of "
+                      f"{cb['leaves']} seeded random leaves (RK2 and concentration on the whole "
+                      f"tree, refit of those leaves); 1 warm-up, median of "
+                      f"{len(cb['times_s'])} timed steps ({cb['median_s']:.2f} s)",
+            "cpu": cpu, "times_s": cb["times_s"],
         }
     print(json.dumps(out))
 

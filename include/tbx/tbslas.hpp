@@ -115,7 +115,12 @@ class Tree {
 };
 
 // Arithmetic of the contraction used by the functors below.
-enum class Arith : uint32_t { kFp64 = TBX_FP64, kExact = TBX_EXACT, kFp32 = TBX_FP32 };
+enum class Arith : uint32_t {
+  kFp64 = TBX_FP64,
+  kExact = TBX_EXACT,
+  kFp32 = TBX_FP32,
+  kFp32Head64 = TBX_FP32_HEAD64
+};
 
 // The reference's functor concept over a device tree.
 template <typename real_t = double>

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import BC_FREESPACE, WRITE_BACK_WRAP, _GridDesc, check, lib, tfac
-from .tree import _init_device, _mode, _ptr, _require_dev, _stream
+from .tree import _check_dep, _init_device, _mode, _ptr, _require_dev, _stream
 
 
 class Grid:
@@ -106,6 +106,7 @@ def grid_semilag_insitu(vel: Grid, conc: Grid, timestep: int, dt: float, num_rk_
                         bc: int = BC_FREESPACE, mode: str = "fp64", g=None, dep=None):
     """One fused in-situ step of grid leaves: samples <- values at the
     departure points of the sample points."""
+    _check_dep(dep, conc.L * conc.P)
     cb = tfac(g)
     check(lib().tbx_grid_semilag_insitu(vel.handle, cb, None, conc.handle, timestep, dt,
                                         num_rk_step, bc, _mode(mode), _ptr(dep), _stream()))

@@ -70,6 +70,18 @@ def _require_dev(t, dtype, name, shape_tail=None):
         raise ValueError(f"{name} must have shape [n,{shape_tail}], got {tuple(t.shape)}")
 
 
+def _check_dep(dep, npts: int) -> None:
+    """A departure-point output buffer, when given, must be a contiguous
+    FP64 cuda tensor [npts][3]: the step kernels write every row of it."""
+    if dep is None:
+        return
+    import torch
+
+    _require_dev(dep, torch.float64, "dep", (3,))
+    if dep.shape[0] != npts:
+        raise ValueError(f"dep must have {npts} rows (one per grid point), got {dep.shape[0]}")
+
+
 def _mode(mode: str) -> int:
     try:
         return MODES[mode]
@@ -314,6 +326,7 @@ def semilag_insitu(vel, conc: Tree, timestep: int, dt: float, num_rk_step: int =
     """SolveSemilagInSitu (tree_semilag.h:92-181): one fused step, conc refit
     in place.  dep (optional cuda tensor [L*P][3]) receives departure points."""
     v = _velocity(vel, g)
+    _check_dep(dep, conc.L * conc.P)
     check(lib().tbx_semilag_insitu_v(v.ref, conc.handle, timestep, dt, num_rk_step, bc,
                                      _mode(mode), _ptr(dep), _stream()))
 
@@ -330,6 +343,7 @@ def semilag_values(vel, conc: Tree, timestep: int, dt: float, num_rk_step: int =
     _require_dev(vals, torch.float64, "vals")
     if vals.numel() != conc.L * conc.dof * conc.P:
         raise ValueError("vals must hold L*dof*P values")
+    _check_dep(dep, conc.L * conc.P)
     check(lib().tbx_semilag_values_v(v.ref, conc.handle, timestep, dt, num_rk_step, bc,
                                      _mode(mode), _ptr(vals), _ptr(dep), _stream()))
     return vals

@@ -162,3 +162,76 @@ def test_tracers_on_matrix_cores(torch, tb, orc, q, dof, bc):
     e = tb.Tree(anchor, depth, q, dof, ccf)
     tb.semilag_insitu(vel, e, 2, dt, 1, bc, "exact")
     assert np.abs(o.get_coeffs() - e.get_coeffs()).max() <= 1e-12 * np.abs(e.get_coeffs()).max()
+
+
+def _bubble(p):
+    """A degree-6 polynomial that vanishes on the faces of the unit cube:
+    represented exactly by every leaf of degree >= 6 and continuous across
+    leaf faces and the domain boundary, where free-space departure points
+    outside the cube evaluate to 0."""
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    return (64 * x * (1 - x) * y * (1 - y) * z * (1 - z))[:, None]
+
+
+@pytest.mark.parametrize("steps", [1, 3])
+def test_batch_against_oracle(torch, tb, orc, steps):
+    # The persistent batch against the oracle's SolveSemilagInSitu on each
+    # problem: degrees 6/8/10 at depths 2 and 3, rigid rotations about random
+    # axes (linear, so every degree represents them exactly), a polynomial
+    # bubble as the concentration.  The batch state is FP32 (coefficients
+    # stored and contracted in FP32, refit summed in FP64): bar 2e-6 of the
+    # largest coefficient, the FP32 rounding of up to 286 terms.  The smooth
+    # field keeps FP32 departure points that cross a leaf face from jumping
+    # by a fit discontinuity.
+    from tbslas_amd import fields
+    from tbslas_amd.batch import Batch
+
+    probs = []
+    u = fields.splitmix64(77, 2 * 9).reshape(9, 2)
+    for p in range(9):
+        q, depth_u = (6, 8, 10)[p % 3], 2 + (p // 3) % 2
+        z, phi = 2 * u[p, 0] - 1, 2 * math.pi * u[p, 1]
+        r = math.sqrt(1 - z * z)
+        axis = (r * math.cos(phi), r * math.sin(phi), z)
+        anchor, depth = tb.uniform_leaves(depth_u)
+        vcf = fields.rotation_coefficients(anchor, depth, q, axis, 2 * math.pi)
+        ccf = tb.fit_function(_bubble, anchor, depth, q, 1)
+        probs.append((anchor, depth, q, vcf, ccf, 0.004 * (1 + p % 4)))
+    vel = [tb.Tree(a, d, q, 3, v) for a, d, q, v, c, dt in probs]
+    conc = [tb.Tree(a, d, q, 1, c) for a, d, q, v, c, dt in probs]
+    b = Batch(vel, conc, [pr[-1] for pr in probs], 0)
+    b.step(steps)
+    for p, (a, d, q, v, c, dt) in enumerate(probs):
+        ov, oc = orc.Tree(a, d, q, 3, v), orc.Tree(a, d, q, 1, c)
+        M = orc.pt2coeff(q)
+        for _ in range(steps):
+            orc.semilag_insitu(ov, oc, M, 1, dt, 1, 0)
+        err = np.abs(b.coeffs(p) - oc.coeff).max() / np.abs(oc.coeff).max()
+        assert err <= 2e-6, (p, q, err)
+    b.close()
+
+
+def test_batch_preserves_a_constant_on_the_largest_problem(torch, tb):
+    # a depth-6, degree-10 problem (262,144 leaves, 348,913,664 departure
+    # points: the largest in the 256-problem batch) beside a small one; a
+    # constant concentration stays constant: every coefficient but c_000
+    # stays 0 and c_000 stays 1, to FP32 rounding
+    from tbslas_amd import fields
+    from tbslas_amd.batch import Batch
+
+    vel, conc = [], []
+    for depth_u, q in ((6, 10), (3, 6)):
+        anchor, depth = tb.uniform_leaves(depth_u)
+        vel.append(tb.Tree(anchor, depth, q, 3,
+                           fields.rotation_coefficients(anchor, depth, q, (0.48, 0.6, 0.64),
+                                                        2 * math.pi)))
+        c = np.zeros((len(depth), 1, tb.ncoef(q)))
+        c[:, 0, 0] = 1.0
+        conc.append(tb.Tree(anchor, depth, q, 1, c))
+    b = Batch(vel, conc, [1.0 / 200, 1.0 / 200], 1)
+    b.step(2)
+    for p in range(2):
+        cf = b.coeffs(p)
+        assert np.abs(cf[:, 0, 0] - 1.0).max() <= 1e-6
+        assert np.abs(cf[:, 0, 1:]).max() <= 1e-6
+    b.close()

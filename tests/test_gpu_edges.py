@@ -118,3 +118,27 @@ def test_odd_component_count_insitu(torch, tb, orc):
         tb.semilag_insitu(gv, gc, n, 0.03, 1, 1, "exact")
         orc.semilag_insitu(ov, oc, M, n, 0.03, 1, 1)
         assert np.abs(gc.get_coeffs() - oc.coeff).max() <= 1e-12 * np.abs(oc.coeff).max(), n
+
+
+def test_departure_buffer_shape_dtype_layout_checked(tb):
+    # wrong dtype, row count or layout of dep raises in the wrapper; the
+    # kernels never see such a buffer
+    import torch
+
+    anchor, depth = tb.uniform_leaves(1)
+    vel, conc = tb.Tree(anchor, depth, 2, 3), tb.Tree(anchor, depth, 2, 1)
+    n = conc.L * conc.P
+    bad = [torch.zeros((n, 3), dtype=torch.float32, device="cuda"),
+           torch.zeros((n - 1, 3), dtype=torch.float64, device="cuda"),
+           torch.zeros((3, n), dtype=torch.float64, device="cuda").t()]
+    for dep in bad:
+        with pytest.raises((TypeError, ValueError)):
+            tb.semilag_insitu(vel, conc, 1, 0.01, 1, 0, "fp64", dep=dep)
+        with pytest.raises((TypeError, ValueError)):
+            tb.semilag_values(vel, conc, 1, 0.01, 1, 0, "fp64", dep=dep)
+    g = tb.Grid(anchor, depth, 4, 1)
+    gv = tb.Grid(anchor, depth, 4, 3)
+    with pytest.raises(ValueError):
+        tb.grid_semilag_insitu(gv, g, 1, 0.01, 1, 0, "fp64",
+                               dep=torch.zeros((g.L * 64 + 1, 3), dtype=torch.float64,
+                                               device="cuda"))

@@ -1,0 +1,99 @@
+"""Parity on the workload the benchmark's headline number is quoted on,
+against the FP64 CPU oracle at full size: the adaptive degree-14 octree
+(36,324 leaves, 122,593,500 grid points), Taylor-Green velocity scaled by
+cos(pi t), periodic, dt = 1e-3 (SolveSemilagInSitu, tree_semilag.h:92-135,
+with SolveSemilagRK2, semilag.inc:26-45).
+
+* One TBX_EXACT step: departure points bit-identical to the oracle's
+  ComputeTrajRK2, their leaf ids bit-identical to the oracle's location,
+  refit coefficients within 1e-12 of the oracle's step (the refit's GEMM
+  sums in another order).
+* Three steps in TBX_FP64 (fused multiply-add): values at every grid point
+  within 1e-12 relative L-inf of three oracle steps.
+* Three steps in TBX_FP32 and TBX_FP32_HEAD64: values at every grid point
+  within 1e-6 relative L-inf of three oracle steps (the FP32-contraction
+  bar; the relative L-inf is max |u - u_oracle| / max |u_oracle|).
+
+The oracle's three steps take about a minute on 16 host threads.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(1200)]
+
+STEPS = 3
+FP64_TOL = 1e-12
+FP32_TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def case(tb, orc):
+    """The workload and the oracle's departure points, leaf ids and states
+    after step 1 and after STEPS steps."""
+    from tbslas_amd import workloads as w
+
+    anchor, depth, ccf, vcf = w.adaptive_q14()
+    q, dt, bc, g = w.ADAPTIVE_Q, w.ADAPTIVE_DT, w.ADAPTIVE_BC, w.taylor_green_tfac
+    ov, oc = orc.Tree(anchor, depth, q, 3, vcf), orc.Tree(anchor, depth, q, 1, ccf)
+    print(f"# headline workload: {len(depth)} leaves, {len(depth) * (q + 1) ** 3} grid points",
+          flush=True)
+    dep = orc.traj_rk2(ov, orc.collect_grid_points(oc), dt, 0.0, 1, bc, g=g)
+    leaf = orc.locate(oc, dep, bc)
+    M = orc.pt2coeff(q)
+    states = []
+    for s in range(1, STEPS + 1):
+        orc.semilag_insitu(ov, oc, M, s, dt, 1, bc, g=g)
+        states.append(oc.coeff.copy())
+        print(f"# oracle step {s} done", flush=True)
+    return dict(anchor=anchor, depth=depth, ccf=ccf, vcf=vcf, q=q, dt=dt, bc=bc, g=g, dep=dep,
+                leaf=leaf, states=states)
+
+
+def test_exact_step_departure_points_leaves_and_refit(torch, tb, case):
+    c = case
+    vel = tb.Tree(c["anchor"], c["depth"], c["q"], 3, c["vcf"])
+    conc = tb.Tree(c["anchor"], c["depth"], c["q"], 1, c["ccf"])
+    dep = torch.empty((conc.L * conc.P, 3), dtype=torch.float64, device="cuda")
+    tb.semilag_insitu(vel, conc, 1, c["dt"], 1, c["bc"], "exact", g=c["g"], dep=dep)
+    got = dep.cpu().numpy()
+    assert np.array_equal(got, c["dep"]), "departure points differ from the oracle"
+    leaf = conc.locate(dep, c["bc"]).cpu().numpy()
+    assert np.array_equal(leaf, c["leaf"]), "leaf ids differ from the oracle"
+    ref = c["states"][0]
+    err = np.abs(conc.get_coeffs() - ref).max() / np.abs(ref).max()
+    assert err <= FP64_TOL, err
+
+
+def _values_at_grid_points(tb, conc, coeff, pts):
+    """Values of conc's tree and of the same leaves holding coeff, both
+    evaluated in TBX_EXACT at pts."""
+    ref = tb.Tree(conc.anchor, conc.depth, conc.q, conc.dof, coeff)
+    a = conc.eval(pts.clone(), 1, "exact")
+    b = ref.eval(pts.clone(), 1, "exact")
+    del ref
+    return a, b
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", FP64_TOL), ("fp32h", FP32_TOL), ("fp32", FP32_TOL)])
+def test_steps_against_oracle(torch, tb, case, mode, tol):
+    c = case
+    vel = tb.Tree(c["anchor"], c["depth"], c["q"], 3, c["vcf"])
+    conc = tb.Tree(c["anchor"], c["depth"], c["q"], 1, c["ccf"])
+    for s in range(1, STEPS + 1):
+        tb.semilag_insitu(vel, conc, s, c["dt"], 1, c["bc"], mode, g=c["g"])
+    pts = conc.grid_points()
+    a, b = _values_at_grid_points(tb, conc, c["states"][-1], pts)
+    err = float((a - b).abs().max() / b.abs().max())
+    print(f"# {mode}: relative L-inf after {STEPS} steps against the oracle {err:.3e}",
+          flush=True)
+    assert err <= tol, err

@@ -192,3 +192,17 @@ def test_step_shape_option(tb):
             _lib.step_shape("fp64", 0)
     finally:
         _lib.set_option(_lib.OPT_STEP_SHAPE, old)
+
+
+def test_departure_buffer_checked_before_any_launch(tb):
+    # a host array or a CPU tensor handed as the departure-point buffer is
+    # refused before the step kernel could write through it
+    import torch
+
+    from tbslas_amd.tree import _check_dep
+
+    _check_dep(None, 10)
+    with pytest.raises(TypeError):
+        _check_dep(np.zeros((10, 3)), 10)
+    with pytest.raises(TypeError):
+        _check_dep(torch.zeros((10, 3), dtype=torch.float64), 10)
