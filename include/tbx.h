@@ -245,6 +245,38 @@ int tbx_grid_semilag_insitu(const tbx_grid *vel, tbx_tfac_fn g, void *gctx, tbx_
                             int timestep, double dt, int num_rk_step, int bc, uint32_t flags,
                             double *dep, tbx_stream s);
 
+/* ---- initial fit and error metric on the device ----------------------
+ * First-kind Chebyshev points of every leaf (the cheb_nodes InitTree
+ * samples the initial field at, tree_utils.h:144): out [L*(q+1)^3][3],
+ * x fastest, leaf after leaf. */
+int tbx_tree_first_kind_points(const tbx_tree *t, double *out, tbx_stream s);
+/* InitTree's fit (the cheb_approx role, tree_utils.h:132-178) on the
+ * device: vals [L*(q+1)^3][dof] point-major at tbx_tree_first_kind_points
+ * -> the tree's coefficients (per-axis discrete Chebyshev transform,
+ * truncated to the triangle; the same as tbx_fit_leaves). */
+int tbx_tree_fit(tbx_tree *t, const double *vals, tbx_stream s);
+/* CheckChebOutput (applications/include/utils.txx:118-297) on the device.
+ * tbx_error_points: the metric's points, n^3 first-kind nodes per leaf with
+ * n = q/3 + 1 (utils.txx:149-153), out [L*n^3][3].  tbx_check_cheb_output:
+ * the tree (evaluated in the mode of flags) against ref [L*n^3][dof], the
+ * exact field at those points (device): the volume-weighted mean error is
+ * removed and res[4] = {absolute L2, relative L2, absolute L-inf,
+ * relative L-inf} (utils.txx:158-293).  dof <= 8. */
+int tbx_error_points(const tbx_tree *t, double *out, tbx_stream s);
+int tbx_check_cheb_output(const tbx_tree *t, const double *ref, uint32_t flags, double *res,
+                          tbx_stream s);
+
+/* ---- output -------------------------------------------------------------
+ * Write2File (the PVFMM tree writer the reference's drivers call,
+ * advection.cpp:184-188, 317-324): the tree's values on order^3 uniform
+ * points per leaf (corners and faces included; each leaf's own polynomial),
+ * as VTK XML hexahedra, order in 2..TBX_MAX_Q + 1, order - 1 cells per axis
+ * per leaf.
+ * Writes <prefix>_000000.vtu (raw appended data, UInt64 headers, point
+ * data "value" with dof components, FP64) and <prefix>.pvtu naming it, the
+ * one-rank form of the reference's per-rank pieces. */
+int tbx_tree_write_vtu(const tbx_tree *t, const char *prefix, int order, tbx_stream s);
+
 /* ---- many small problems in one persistent launch --------------------
  * A batch advances nprob independent problems (velocity tree vel[p], dof 3;
  * concentration tree conc[p], dof 1; degree 6, 8 or 10; time step dt[p];
@@ -316,6 +348,11 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
 int tbx_new_nodes(int q, double *x);
 /* GetPt2CoeffMatrix (cheb.h:166-196): M[P][Nc] = pinv(B)^T. */
 int tbx_pt2coeff_matrix(int q, double *M);
+/* The same operator in the factored form the device refit applies:
+ * pinv(B) = H B^T with H = (B^T B)^-1, H [Nc][Nc] (symmetric).  B is a
+ * tensor product of one-axis bases restricted to the triangle, so B^T runs
+ * as three one-axis contractions and H as one GEMM over all leaves. */
+int tbx_refit_gram_inverse(int q, double *H);
 /* Uniform tree of the given depth in Morton order: anchor [8^depth][3],
  * depth [8^depth]. */
 int tbx_uniform_leaves(int depth, uint32_t *anchor, uint8_t *dep);

@@ -9,13 +9,14 @@ from ._lib import (BC_FREESPACE, BC_PERIODIC, EXACT, FP32, FP64, MAX_DEPTH, MAX_
                    lib)
 from .grid import Grid, grid_semilag_insitu, sample_function
 from .tree import (Tree, Velocity, adaptive_leaves, balance21, first_kind_points, fit_function,
-                   fit_leaves, ncoef, new_nodes, pt2coeff_matrix, semilag_insitu, semilag_rk2,
-                   semilag_values, tail_norm, traj_rk2, uniform_leaves)
+                   fit_leaves, ncoef, new_nodes, pt2coeff_matrix, refit_gram_inverse, semilag_insitu,
+                   semilag_rk2, semilag_values, tail_norm, traj_rk2, uniform_leaves)
 
 __all__ = [
     "BC_FREESPACE", "BC_PERIODIC", "EXACT", "FP32", "FP64", "Grid", "MAX_DEPTH", "MAX_Q",
     "TbxError", "grid_semilag_insitu", "sample_function",
     "Tree", "Velocity", "adaptive_leaves", "balance21", "first_kind_points", "fit_function",
-    "fit_leaves", "lib", "ncoef", "new_nodes", "pt2coeff_matrix", "semilag_insitu",
+    "fit_leaves", "lib", "ncoef", "new_nodes", "pt2coeff_matrix", "refit_gram_inverse",
+    "semilag_insitu",
     "semilag_rk2", "semilag_values", "tail_norm", "traj_rk2", "uniform_leaves",
 ]

@@ -110,6 +110,11 @@ _SIGS = {
     "tbx_grid_get_samples": (_I32, [_P, _P, _I32, _P]),
     "tbx_grid_eval": (_I32, [_P, _P, _I64, _I32, _U32, _P, _P]),
     "tbx_grid_semilag_insitu": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P]),
+    "tbx_tree_first_kind_points": (_I32, [_P, _P, _P]),
+    "tbx_tree_fit": (_I32, [_P, _P, _P]),
+    "tbx_error_points": (_I32, [_P, _P, _P]),
+    "tbx_check_cheb_output": (_I32, [_P, _P, _U32, _P, _P]),
+    "tbx_tree_write_vtu": (_I32, [_P, C.c_char_p, _I32, _P]),
     "tbx_batch_create": (_I32, [_I32, _P, _P, _P, _I32, C.POINTER(_P)]),
     "tbx_batch_destroy": (_I32, [_P]),
     "tbx_batch_set_config": (_I32, [_P, _I32]),
@@ -119,6 +124,7 @@ _SIGS = {
     "tbx_batch_get_coeffs": (_I32, [_P, _I32, _P, _P]),
     "tbx_new_nodes": (_I32, [_I32, _P]),
     "tbx_pt2coeff_matrix": (_I32, [_I32, _P]),
+    "tbx_refit_gram_inverse": (_I32, [_I32, _P]),
     "tbx_uniform_leaves": (_I32, [_I32, _P, _P]),
     "tbx_first_kind_points": (_I32, [_I64, _P, _P, _I32, _P]),
     "tbx_fit_leaves": (_I32, [_I64, _I32, _I32, _P, _P]),

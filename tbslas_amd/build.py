@@ -61,6 +61,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     units.append((os.path.join(CSRC, "tbx_common.hip"), os.path.join(BUILD, "tbx_common.o"), []))
     units.append((os.path.join(CSRC, "tbx_grid.hip"), os.path.join(BUILD, "tbx_grid.o"), []))
     units.append((os.path.join(CSRC, "tbx_batch.hip"), os.path.join(BUILD, "tbx_batch.o"), []))
+    units.append((os.path.join(CSRC, "tbx_refit.hip"), os.path.join(BUILD, "tbx_refit.o"), []))
     units.append((os.path.join(CSRC, "tbx_host.cpp"), os.path.join(BUILD, "tbx_host.o"),
                   ["-x", "hip"]))
 
@@ -76,7 +77,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
         objs = list(ex.map(compile_unit, units))
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-          "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

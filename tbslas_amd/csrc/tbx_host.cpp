@@ -4,7 +4,6 @@
 // directory, coefficients and their FP32 mirror), checks arguments, picks
 // the per-degree kernels and runs the refit GEMM.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <atomic>
@@ -126,6 +125,12 @@ void cheb_poly_host(int q, double x, double *p) {
   }
 }
 
+// n first-kind Chebyshev nodes on [0, 1] (the cheb_nodes InitTree samples
+// at, tree_utils.h:144).
+void first_kind_nodes_host(int n, double *x) {
+  for (int i = 0; i < n; ++i) x[i] = -std::cos((i + 0.5) * M_PI / n) * 0.5 + 0.5;
+}
+
 void new_nodes_host(int q, double *x) {
   const unsigned d = (unsigned)q + 1;
   const double scal = 1.0 / std::cos(0.5 * M_PI / d);
@@ -133,20 +138,32 @@ void new_nodes_host(int q, double *x) {
     x[i] = -std::cos((i + 0.5) * M_PI / d) * scal * 0.5 + 0.5;
 }
 
-// M = pinv(B)^T by Householder QR of B (in long double): with B = Q R,
-// M = Q R^-T.  B[p][m] is basis m (packed i,j,k) at stretched node p mapped
-// to [-1,1] (cheb.h:170-180).
-std::vector<double> build_pt2coeff(int q) {
-  typedef long double R_t;
-  const int d = q + 1, P = d * d * d, nc = ncoef(q);
-  std::vector<double> x(d), xi(d), T((size_t)d * d);
+// Householder QR of B (P x nc, column-major in A on entry, in long double),
+// B[p][m] = basis m (packed i,j,k) at stretched node p mapped to [-1,1]
+// (cheb.h:170-180): returns R (nc x nc, row-major upper triangle) and the
+// reflectors (V, beta).
+typedef long double R_t;
+struct Householder {
+  int P = 0, nc = 0;
+  std::vector<R_t> R, V, beta;
+};
+
+std::vector<double> stretched_basis_1d(int q) {
+  // T[k * d + a] = T_k(node a mapped to [-1, 1])
+  const int d = q + 1;
+  std::vector<double> x(d), T((size_t)d * d);
   new_nodes_host(q, x.data());
-  for (int i = 0; i < d; ++i) xi[i] = x[i] * 2.0 - 1.0;
-  for (int i = 0; i < d; ++i) {
+  for (int a = 0; a < d; ++a) {
     std::vector<double> p(d);
-    cheb_poly_host(q, xi[i], p.data());
-    for (int k = 0; k < d; ++k) T[(size_t)k * d + i] = p[k];
+    cheb_poly_host(q, x[a] * 2.0 - 1.0, p.data());
+    for (int k = 0; k < d; ++k) T[(size_t)k * d + a] = p[k];
   }
+  return T;
+}
+
+Householder householder_basis(int q) {
+  const int d = q + 1, P = d * d * d, nc = ncoef(q);
+  const std::vector<double> T = stretched_basis_1d(q);
   // A column-major P x nc
   std::vector<R_t> A((size_t)P * nc);
   {
@@ -161,40 +178,52 @@ std::vector<double> build_pt2coeff(int q) {
           }
         }
   }
-  std::vector<R_t> R((size_t)nc * nc, 0.0), V((size_t)P * nc, 0.0), beta(nc);
+  Householder h;
+  h.P = P;
+  h.nc = nc;
+  h.R.assign((size_t)nc * nc, 0.0);
+  h.V.assign((size_t)P * nc, 0.0);
+  h.beta.assign(nc, 0.0);
   for (int c = 0; c < nc; ++c) {
     R_t *a = &A[(size_t)c * P];
     R_t nrm = 0;
     for (int p = c; p < P; ++p) nrm += a[p] * a[p];
     nrm = std::sqrt(nrm);
     const R_t alpha = a[c] > 0 ? -nrm : nrm;
-    R_t *v = &V[(size_t)c * P];
+    R_t *v = &h.V[(size_t)c * P];
     for (int p = c; p < P; ++p) v[p] = a[p];
     v[c] -= alpha;
     R_t vv = 0;
     for (int p = c; p < P; ++p) vv += v[p] * v[p];
-    beta[c] = vv > 0 ? 2.0L / vv : 0.0L;
-    R[(size_t)c * nc + c] = alpha;
+    h.beta[c] = vv > 0 ? 2.0L / vv : 0.0L;
+    h.R[(size_t)c * nc + c] = alpha;
     parallel_for(nc - c - 1, [&](int64_t jj) {
       const int j = c + 1 + (int)jj;
       R_t *aj = &A[(size_t)j * P];
       R_t s = 0;
       for (int p = c; p < P; ++p) s += v[p] * aj[p];
-      s *= beta[c];
+      s *= h.beta[c];
       for (int p = c; p < P; ++p) aj[p] -= s * v[p];
     });
-    for (int j = c + 1; j < nc; ++j) R[(size_t)c * nc + j] = A[(size_t)j * P + c];
+    for (int j = c + 1; j < nc; ++j) h.R[(size_t)c * nc + j] = A[(size_t)j * P + c];
   }
+  return h;
+}
+
+// M = pinv(B)^T = Q R^-T (GetPt2CoeffMatrix, cheb.h:166-196), M[P][nc].
+std::vector<double> build_pt2coeff(int q) {
+  const Householder h = householder_basis(q);
+  const int P = h.P, nc = h.nc;
   // Q (P x nc, column-major) = H_0 ... H_{nc-1} [I; 0]
   std::vector<R_t> Q((size_t)P * nc, 0.0);
   for (int c = 0; c < nc; ++c) Q[(size_t)c * P + c] = 1.0;
   parallel_for(nc, [&](int64_t col) {
     R_t *qc = &Q[(size_t)col * P];
     for (int c = nc - 1; c >= 0; --c) {
-      const R_t *v = &V[(size_t)c * P];
+      const R_t *v = &h.V[(size_t)c * P];
       R_t s = 0;
       for (int p = c; p < P; ++p) s += v[p] * qc[p];
-      s *= beta[c];
+      s *= h.beta[c];
       if (s != 0)
         for (int p = c; p < P; ++p) qc[p] -= s * v[p];
     }
@@ -206,12 +235,102 @@ std::vector<double> build_pt2coeff(int q) {
     for (int a = 0; a < nc; ++a) r[a] = Q[(size_t)a * P + p];
     for (int a = nc - 1; a >= 0; --a) {
       R_t s = r[a];
-      for (int k = a + 1; k < nc; ++k) s -= R[(size_t)a * nc + k] * r[k];
-      r[a] = s / R[(size_t)a * nc + a];
+      for (int k = a + 1; k < nc; ++k) s -= h.R[(size_t)a * nc + k] * r[k];
+      r[a] = s / h.R[(size_t)a * nc + a];
     }
     for (int a = 0; a < nc; ++a) M[(size_t)p * nc + a] = (double)r[a];
   });
   return M;
+}
+
+// The refit operator in factored form: pinv(B) = H B^T with
+// H = (B^T B)^-1.  H is stored
+// zero-padded to [KP][NP] (KP a multiple of 16, NP of 64) for the GEMM, and
+// the one-axis basis T_k(node a) for the tensor-product contractions.
+struct RefitOperator {
+  int KP = 0, NP = 0;
+  std::vector<double> H;  // [KP][NP]
+  RefitTable tab{};
+};
+
+RefitOperator build_refit_operator(int q) {
+  // B = Tx (x) Ty (x) Tz on the triangle's columns, so the Gram matrix is
+  // B^T B [m][m'] = G[k][k'] G[j][j'] G[i][i'], G = T T^T (d x d); H is its
+  // inverse by Cholesky in long double (B^T B is symmetric positive
+  // definite with condition number ~14 at every degree built here).
+  const int d = q + 1, nc = ncoef(q);
+  const std::vector<double> T = stretched_basis_1d(q);
+  std::vector<R_t> G((size_t)d * d);
+  for (int a = 0; a < d; ++a)
+    for (int b = 0; b < d; ++b) {
+      R_t s = 0;
+      for (int n = 0; n < d; ++n) s += (R_t)T[(size_t)a * d + n] * T[(size_t)b * d + n];
+      G[(size_t)a * d + b] = s;
+    }
+  std::vector<int> ti(nc), tj(nc), tk(nc);
+  {
+    int m = 0;
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; i + j < d; ++j)
+        for (int k = 0; i + j + k < d; ++k, ++m) {
+          ti[m] = i;
+          tj[m] = j;
+          tk[m] = k;
+        }
+  }
+  // Cholesky A = L L^T, L lower triangular (row-major, in place in A)
+  std::vector<R_t> A((size_t)nc * nc);
+  parallel_for(nc, [&](int64_t a) {
+    for (int b = 0; b < nc; ++b)
+      A[(size_t)a * nc + b] = G[(size_t)tk[a] * d + tk[b]] * G[(size_t)tj[a] * d + tj[b]] *
+                              G[(size_t)ti[a] * d + ti[b]];
+  });
+  for (int c = 0; c < nc; ++c) {
+    R_t s = A[(size_t)c * nc + c];
+    for (int k = 0; k < c; ++k) s -= A[(size_t)c * nc + k] * A[(size_t)c * nc + k];
+    const R_t lcc = std::sqrt(s);
+    A[(size_t)c * nc + c] = lcc;
+    parallel_for(nc - c - 1, [&](int64_t rr) {
+      const int r = c + 1 + (int)rr;
+      R_t t = A[(size_t)r * nc + c];
+      for (int k = 0; k < c; ++k) t -= A[(size_t)r * nc + k] * A[(size_t)c * nc + k];
+      A[(size_t)r * nc + c] = t / lcc;
+    });
+  }
+  // Linv (lower triangular), column by column: L Linv = I
+  std::vector<R_t> Li((size_t)nc * nc, 0.0);
+  parallel_for(nc, [&](int64_t col) {
+    for (int r = (int)col; r < nc; ++r) {
+      R_t s = r == col ? 1.0L : 0.0L;
+      for (int k = (int)col; k < r; ++k) s -= A[(size_t)r * nc + k] * Li[(size_t)k * nc + col];
+      Li[(size_t)r * nc + col] = s / A[(size_t)r * nc + r];
+    }
+  });
+  RefitOperator op;
+  op.KP = (nc + 15) / 16 * 16;
+  op.NP = (nc + 63) / 64 * 64;
+  op.H.assign((size_t)op.KP * op.NP, 0.0);
+  // H = L^-T L^-1: H[a][b] = sum_k Linv[k][a] Linv[k][b], k >= max(a, b)
+  parallel_for(nc, [&](int64_t a) {
+    for (int b = 0; b < nc; ++b) {
+      R_t s = 0;
+      for (int k = std::max((int)a, b); k < nc; ++k)
+        s += Li[(size_t)k * nc + a] * Li[(size_t)k * nc + b];
+      op.H[(size_t)a * op.NP + b] = (double)s;
+    }
+  });
+  for (int k = 0; k < d; ++k)
+    for (int a = 0; a < d; ++a) op.tab.T[k][a] = T[(size_t)k * d + a];
+  return op;
+}
+
+const RefitOperator &refit_operator_cached(int q) {
+  static std::mutex mu;
+  static std::map<int, RefitOperator> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(q);
+  if (it == cache.end()) it = cache.emplace(q, build_refit_operator(q)).first;
+  return it->second;
 }
 
 const std::vector<double> &pt2coeff_cached(int q) {
@@ -235,13 +354,14 @@ struct tbx_tree_s {
   uint4 *d_info = nullptr;
   int32_t *d_dir = nullptr;
   double *d_coeff = nullptr;
-  double *d_coeff_next = nullptr;  // refit target, swapped after the GEMM
   float *d_coeff32 = nullptr;
   bool f32_stale = true;
   double *d_vals = nullptr;  // [L][dof][P] values at the grid points
-  double *d_M = nullptr;     // [P][nc] refit operator
+  double *d_H = nullptr;     // [KP][NP] refit operator (B^T B)^-1, zero-padded
+  double *d_W = nullptr;     // [L*dof][KP] refit workspace (B^T values)
+  int KP = 0, NP = 0;
+  RefitTable tab{};
   double *d_adjA = nullptr;  // [L*P][3] per-point velocity cotangent factors
-  rocblas_handle blas = nullptr;
 
   TreeView view() const {
     TreeView v{};
@@ -270,12 +390,11 @@ void free_tree(tbx_tree *t) {
   hipFree(t->d_info);
   hipFree(t->d_dir);
   hipFree(t->d_coeff);
-  hipFree(t->d_coeff_next);
   hipFree(t->d_coeff32);
   hipFree(t->d_vals);
-  hipFree(t->d_M);
+  hipFree(t->d_H);
+  hipFree(t->d_W);
   hipFree(t->d_adjA);
-  if (t->blas) rocblas_destroy_handle(t->blas);
   delete t;
 }
 
@@ -477,40 +596,35 @@ tbx_velocity single_tree(const tbx_tree *t, tbx_tfac_fn g, void *gctx) {
   return v;
 }
 
+// The refit operator on the device (once per tree) and the workspace.
 int ensure_refit(tbx_tree *t, hipStream_t s) {
-  const int P = t->P();
-  if (!t->d_M) {
-    const std::vector<double> &M = pt2coeff_cached(t->q);
-    HIP_TRY(hipMalloc(&t->d_M, sizeof(double) * M.size()));
-    HIP_TRY(hipMemcpy(t->d_M, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice));
+  if (!t->d_H) {
+    const RefitOperator &op = refit_operator_cached(t->q);
+    HIP_TRY(hipMalloc(&t->d_H, sizeof(double) * op.H.size()));
+    HIP_TRY(hipMemcpy(t->d_H, op.H.data(), sizeof(double) * op.H.size(), hipMemcpyHostToDevice));
+    t->KP = op.KP;
+    t->NP = op.NP;
+    t->tab = op.tab;
   }
-  if (!t->d_coeff_next)
-    HIP_TRY(hipMalloc(&t->d_coeff_next, sizeof(double) * (size_t)t->ncoeff_total() + kCoeffPad));
-  if (!t->blas) {
-    if (rocblas_create_handle(&t->blas) != rocblas_status_success)
-      return fail(TBX_EHIP, "rocblas_create_handle failed");
-  }
-  if (rocblas_set_stream(t->blas, s) != rocblas_status_success)
-    return fail(TBX_EHIP, "rocblas_set_stream failed");
-  (void)P;
+  if (!t->d_W) HIP_TRY(hipMalloc(&t->d_W, sizeof(double) * (size_t)t->L * t->dof * t->KP));
+  (void)s;
   return TBX_OK;
 }
 
-// coeff_next[r][m] = sum_p vals[r][p] M[p][m], r over L*dof rows, then swap
-// (SetTreeGridValues GEMM, tree_utils.h:534-546).
+// coeff = pinv(B) vals for every leaf component (SetTreeGridValues,
+// tree_utils.h:534-546): W = B^T vals by the one-axis contractions, then
+// coeff = W H on the matrix cores, written in place together with the FP32
+// mirror when the tree has one.
 int refit(tbx_tree *t, const double *vals, hipStream_t s) {
   int rc = ensure_refit(t, s);
   if (rc) return rc;
-  const int P = t->P(), nc = t->nc;
   const int64_t rows = t->L * t->dof;
   if (rows > INT32_MAX) return fail(TBX_EOVERFLOW, "refit: %lld rows", (long long)rows);
-  const double one = 1.0, zero = 0.0;
-  rocblas_status st = rocblas_dgemm(t->blas, rocblas_operation_none, rocblas_operation_none,
-                                    nc, (rocblas_int)rows, P, &one, t->d_M, nc, vals, P,
-                                    &zero, t->d_coeff_next, nc);
-  if (st != rocblas_status_success) return fail(TBX_EHIP, "rocblas_dgemm failed (%d)", (int)st);
-  std::swap(t->d_coeff, t->d_coeff_next);
-  t->f32_stale = true;
+  HIP_TRY(launch_refit_project(t->q + 1, vals, rows, t->dof, t->P(), 1, t->tab, t->d_W, t->KP,
+                               s));
+  HIP_TRY(launch_refit_gemm(t->d_W, t->KP, rows, t->KP, t->KP, t->d_H, t->NP, t->d_coeff, t->nc,
+                            t->nc, t->d_coeff32, s));
+  t->f32_stale = t->d_coeff32 == nullptr;
   return TBX_OK;
 }
 
@@ -954,12 +1068,12 @@ int adj_conc_step(const AdjCtx &c, tbx_tree *conc, int timestep, double dt, int 
   const int P = conc->P(), nc = conc->nc;
   const int64_t rows = conc->L * conc->dof;
   if (rows > INT32_MAX) return fail(TBX_EOVERFLOW, "adjoint: %lld rows", (long long)rows);
-  // R^T: ua[r][p] = sum_m lam[r][m] M[p][m]
-  const double one = 1.0, zero = 0.0;
-  rocblas_status bs = rocblas_dgemm(conc->blas, rocblas_operation_transpose,
-                                    rocblas_operation_none, P, (rocblas_int)rows, nc, &one,
-                                    conc->d_M, nc, lam_next, nc, &zero, conc->d_vals, P);
-  if (bs != rocblas_status_success) return fail(TBX_EHIP, "rocblas_dgemm failed (%d)", (int)bs);
+  // R^T: ua = pinv(B)^T lam = B (H lam) per row
+  HIP_TRY(launch_refit_gemm(lam_next, nc, rows, nc, conc->KP, conc->d_H, conc->NP, conc->d_W,
+                            conc->KP, nc, nullptr, st));
+  HIP_TRY(launch_refit_expand(conc->q + 1, conc->d_W, rows, conc->tab, conc->KP, conc->d_vals,
+                              st));
+  (void)P;
   HIP_TRY(hipMemsetAsync(lam_out, 0, sizeof(double) * (size_t)conc->ncoeff_total(), st));
   StepArgs a;
   int rc = adj_args(c, conc, timestep, dt, bc, kAdjBlock, &a);
@@ -1031,7 +1145,7 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
   const size_t ndep = stationary ? 1 : (size_t)(K > 1 ? K - 1 : 0);
   const bool keep_dep = ndep > 0 && ndep * dep_bytes <= free_mem / 2;
   std::vector<double *> ck(nck, nullptr), seg(K, nullptr), deps(keep_dep ? ndep : 0, nullptr);
-  double *lam_a = nullptr, *lam_b = nullptr, *cT = nullptr;
+  double *lam_a = nullptr, *lam_b = nullptr, *cT = nullptr, *partial = nullptr;
   auto release = [&]() {
     for (double *p : ck) hipFree(p);
     for (double *p : seg) hipFree(p);
@@ -1039,6 +1153,7 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
     hipFree(lam_a);
     hipFree(lam_b);
     hipFree(cT);
+    hipFree(partial);
   };
   auto body = [&]() -> int {
     for (auto &p : ck) HIP_TRY(hipMalloc(&p, bytes));
@@ -1047,6 +1162,7 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
     HIP_TRY(hipMalloc(&lam_a, bytes));
     HIP_TRY(hipMalloc(&lam_b, bytes));
     HIP_TRY(hipMalloc(&cT, bytes));
+    HIP_TRY(hipMalloc(&partial, sizeof(double) * diff_norm2_blocks((int64_t)nco)));
     const tbx_velocity &v = c.v;
     // One forward step.  With a stationary velocity every step has step 1's
     // departure points: once they are kept, a one-component step is the
@@ -1076,16 +1192,16 @@ int tbx_adjoint_run(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *co
     }
     HIP_TRY(hipMemcpyAsync(cT, conc->d_coeff, bytes, hipMemcpyDeviceToDevice, st));
     // lam_T = c_T - c*, loss = 0.5 |lam_T|^2
-    HIP_TRY(hipMemcpyAsync(lam_a, cT, bytes, hipMemcpyDeviceToDevice, st));
-    const double m1 = -1.0;
-    if (rocblas_daxpy(conc->blas, (rocblas_int)nco, &m1, c_target, 1, lam_a, 1) !=
-        rocblas_status_success)
-      return fail(TBX_EHIP, "rocblas_daxpy failed");
-    double dot = 0.0;
-    if (rocblas_ddot(conc->blas, (rocblas_int)nco, lam_a, 1, lam_a, 1, &dot) !=
-        rocblas_status_success)
-      return fail(TBX_EHIP, "rocblas_ddot failed");
-    if (loss) *loss = 0.5 * dot;
+    {
+      const int nb = diff_norm2_blocks(nco);
+      HIP_TRY(launch_diff_norm2(cT, c_target, nco, lam_a, partial, st));
+      std::vector<double> h(nb);
+      HIP_TRY(hipMemcpyAsync(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      double dot = 0.0;
+      for (double v : h) dot += v;
+      if (loss) *loss = 0.5 * dot;
+    }
     // backward sweep by segments, recomputing c_n from the checkpoints
     if (grad_vel) {
       HIP_TRY(hipMemsetAsync(grad_vel, 0, sizeof(double) * (size_t)vel->L * 3 * vel->nc, st));
@@ -1600,6 +1716,15 @@ int tbx_pt2coeff_matrix(int q, double *M) {
   return TBX_OK;
 }
 
+int tbx_refit_gram_inverse(int q, double *H) {
+  if (q < 1 || q > TBX_MAX_Q || !H) return fail(TBX_EINVAL, "tbx_refit_gram_inverse: bad argument");
+  const RefitOperator &op = refit_operator_cached(q);
+  const int nc = ncoef(q);
+  for (int a = 0; a < nc; ++a)
+    for (int b = 0; b < nc; ++b) H[(size_t)a * nc + b] = op.H[(size_t)a * op.NP + b];
+  return TBX_OK;
+}
+
 int tbx_uniform_leaves(int depth, uint32_t *anchor, uint8_t *dep) {
   if (depth < 0 || depth > 10 || !anchor || !dep)
     return fail(TBX_EINVAL, "tbx_uniform_leaves: depth %d", depth);
@@ -1679,6 +1804,106 @@ int tbx_fit_leaves(int64_t L, int q, int dof, const double *vals, double *coeff)
         for (int k = 0; i + jj + k < d; ++k) c[m++] = b[((size_t)i * d + jj) * d + k];
   });
   return TBX_OK;
+}
+
+namespace {
+
+// C[k][a] = (2/d) T_k(xi_a), halved for k = 0, xi_a the first-kind nodes on
+// [-1, 1]: the one-axis discrete Chebyshev transform of the initial fit
+// (tbx_fit_leaves' table).
+RefitTable fit_table(int q) {
+  const int d = q + 1;
+  RefitTable tab{};
+  double x[TBX_MAX_Q + 1];
+  first_kind_nodes_host(d, x);
+  for (int a = 0; a < d; ++a) {
+    double p[TBX_MAX_Q + 1];
+    cheb_poly_host(q, x[a] * 2.0 - 1.0, p);
+    for (int k = 0; k < d; ++k) tab.T[k][a] = p[k] * (2.0 / d) * (k == 0 ? 0.5 : 1.0);
+  }
+  return tab;
+}
+
+}  // namespace
+
+int tbx_tree_first_kind_points(const tbx_tree *t, double *out, tbx_stream s) {
+  if (!t || !out) return fail(TBX_EINVAL, "tbx_tree_first_kind_points: null argument");
+  double x[TBX_MAX_Q + 1];
+  first_kind_nodes_host(t->q + 1, x);
+  HIP_TRY(launch_grid_points(t->view(), x, t->q + 1, out, (hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_tree_fit(tbx_tree *t, const double *vals, tbx_stream s) {
+  if (!t || !vals) return fail(TBX_EINVAL, "tbx_tree_fit: null argument");
+  const int64_t rows = t->L * t->dof;
+  if (rows > INT32_MAX) return fail(TBX_EOVERFLOW, "tbx_tree_fit: %lld rows", (long long)rows);
+  HIP_TRY(launch_refit_project(t->q + 1, vals, rows, t->dof, 1, t->dof, fit_table(t->q),
+                               t->d_coeff, t->nc, (hipStream_t)s));
+  t->f32_stale = true;
+  return TBX_OK;
+}
+
+int tbx_error_points(const tbx_tree *t, double *out, tbx_stream s) {
+  if (!t || !out) return fail(TBX_EINVAL, "tbx_error_points: null argument");
+  const int n = t->q / 3 + 1;
+  double x[TBX_MAX_Q + 1];
+  first_kind_nodes_host(n, x);
+  HIP_TRY(launch_grid_points(t->view(), x, n, out, (hipStream_t)s));
+  return TBX_OK;
+}
+
+int tbx_check_cheb_output(const tbx_tree *tc, const double *ref, uint32_t flags, double *res,
+                          tbx_stream s) {
+  if (!tc || !ref || !res) return fail(TBX_EINVAL, "tbx_check_cheb_output: null argument");
+  tbx_tree *t = const_cast<tbx_tree *>(tc);
+  if (t->dof > 8) return fail(TBX_EUNSUPPORTED, "tbx_check_cheb_output: dof %d > 8", t->dof);
+  hipStream_t st = (hipStream_t)s;
+  int mode, rc;
+  if ((rc = mode_of(flags, &mode))) return rc;
+  if ((rc = prepare_mode(t, mode, st))) return rc;
+  const int n = t->q / 3 + 1, dof = t->dof;
+  const int64_t npts = t->L * (int64_t)n * n * n;
+  const int nb = metric_blocks(npts);
+  double *pts = nullptr, *val = nullptr, *part = nullptr;
+  auto body = [&]() -> int {
+    HIP_TRY(hipMallocAsync((void **)&pts, sizeof(double) * 3 * (size_t)npts, st));
+    HIP_TRY(hipMallocAsync((void **)&val, sizeof(double) * dof * (size_t)npts, st));
+    HIP_TRY(hipMallocAsync((void **)&part, sizeof(double) * nb * std::max(dof, 4), st));
+    double x[TBX_MAX_Q + 1];
+    first_kind_nodes_host(n, x);
+    HIP_TRY(launch_grid_points(t->view(), x, n, pts, st));
+    HIP_TRY(dispatch_eval(t->q, mode, t->view(), pts, npts, TBX_BC_FREESPACE, 0, val, st));
+    std::vector<double> h((size_t)nb * std::max(dof, 4));
+    // pass 0: the volume-weighted mean error per component (utils.txx:158-187)
+    HIP_TRY(launch_metric(t->d_info, n, val, ref, npts, dof, nullptr, 0, part, st));
+    HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * nb * dof, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double avg[8] = {};
+    for (int b = 0; b < nb; ++b)
+      for (int l = 0; l < dof; ++l) avg[l] += h[(size_t)b * dof + l];
+    // pass 1: L2 and L-inf of the field and of the centred error (:240-293)
+    HIP_TRY(launch_metric(t->d_info, n, val, ref, npts, dof, avg, 1, part, st));
+    HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * nb * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double l2 = 0, l2e = 0, mx = 0, mxe = 0;
+    for (int b = 0; b < nb; ++b) {
+      l2 += h[(size_t)b * 4 + 0];
+      l2e += h[(size_t)b * 4 + 1];
+      mx = std::max(mx, h[(size_t)b * 4 + 2]);
+      mxe = std::max(mxe, h[(size_t)b * 4 + 3]);
+    }
+    res[0] = std::sqrt(l2e);
+    res[1] = std::sqrt(l2e / l2);
+    res[2] = mxe;
+    res[3] = mxe / mx;
+    return TBX_OK;
+  };
+  rc = body();
+  hipFreeAsync(pts, st);
+  hipFreeAsync(val, st);
+  hipFreeAsync(part, st);
+  return rc;
 }
 
 int tbx_tail_norm(int64_t L, int q, int dof, const double *coeff, double *tail) {
@@ -1776,6 +2001,164 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor, uint8
   }
   *num_leaves = L;
   return TBX_OK;
+}
+
+// Write2File: the tree on order^3 uniform points per leaf as VTK XML
+// hexahedra (one .vtu piece plus the .pvtu that names it).
+namespace {
+
+struct AppendedBlock {
+  const void *data;
+  uint64_t bytes;
+};
+
+int write_vtu_files(const std::string &prefix, int dof, int64_t npts, int64_t ncells,
+                    const std::vector<double> &xyz, const std::vector<double> &val,
+                    const std::vector<int64_t> &conn, const std::vector<int64_t> &offs,
+                    const std::vector<uint8_t> &types) {
+  const std::string piece = prefix + "_000000.vtu";
+  const size_t slash = piece.find_last_of('/');
+  const std::string piece_name = slash == std::string::npos ? piece : piece.substr(slash + 1);
+  const AppendedBlock blocks[5] = {{val.data(), sizeof(double) * val.size()},
+                                   {xyz.data(), sizeof(double) * xyz.size()},
+                                   {conn.data(), sizeof(int64_t) * conn.size()},
+                                   {offs.data(), sizeof(int64_t) * offs.size()},
+                                   {types.data(), types.size()}};
+  uint64_t off[5], o = 0;
+  for (int b = 0; b < 5; ++b) {
+    off[b] = o;
+    o += sizeof(uint64_t) + blocks[b].bytes;
+  }
+  FILE *f = std::fopen(piece.c_str(), "wb");
+  if (!f) return fail(TBX_EINVAL, "tbx_tree_write_vtu: cannot open %s", piece.c_str());
+  std::fprintf(f,
+               "<?xml version=\"1.0\"?>\n"
+               "<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
+               "header_type=\"UInt64\">\n"
+               " <UnstructuredGrid>\n"
+               "  <Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n"
+               "   <PointData Scalars=\"value\">\n"
+               "    <DataArray type=\"Float64\" Name=\"value\" NumberOfComponents=\"%d\" "
+               "format=\"appended\" offset=\"%llu\"/>\n"
+               "   </PointData>\n"
+               "   <Points>\n"
+               "    <DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"appended\" "
+               "offset=\"%llu\"/>\n"
+               "   </Points>\n"
+               "   <Cells>\n"
+               "    <DataArray type=\"Int64\" Name=\"connectivity\" format=\"appended\" "
+               "offset=\"%llu\"/>\n"
+               "    <DataArray type=\"Int64\" Name=\"offsets\" format=\"appended\" "
+               "offset=\"%llu\"/>\n"
+               "    <DataArray type=\"UInt8\" Name=\"types\" format=\"appended\" "
+               "offset=\"%llu\"/>\n"
+               "   </Cells>\n"
+               "  </Piece>\n"
+               " </UnstructuredGrid>\n"
+               " <AppendedData encoding=\"raw\">\n_",
+               (long long)npts, (long long)ncells, dof, (unsigned long long)off[0],
+               (unsigned long long)off[1], (unsigned long long)off[2], (unsigned long long)off[3],
+               (unsigned long long)off[4]);
+  bool ok = true;
+  for (int b = 0; b < 5 && ok; ++b) {
+    ok = std::fwrite(&blocks[b].bytes, sizeof(uint64_t), 1, f) == 1;
+    if (ok && blocks[b].bytes)
+      ok = std::fwrite(blocks[b].data, 1, blocks[b].bytes, f) == blocks[b].bytes;
+  }
+  std::fprintf(f, "\n </AppendedData>\n</VTKFile>\n");
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(TBX_EINVAL, "tbx_tree_write_vtu: write to %s failed", piece.c_str());
+  const std::string pv = prefix + ".pvtu";
+  f = std::fopen(pv.c_str(), "w");
+  if (!f) return fail(TBX_EINVAL, "tbx_tree_write_vtu: cannot open %s", pv.c_str());
+  std::fprintf(f,
+               "<?xml version=\"1.0\"?>\n"
+               "<VTKFile type=\"PUnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
+               "header_type=\"UInt64\">\n"
+               " <PUnstructuredGrid GhostLevel=\"0\">\n"
+               "  <PPointData Scalars=\"value\">\n"
+               "   <PDataArray type=\"Float64\" Name=\"value\" NumberOfComponents=\"%d\"/>\n"
+               "  </PPointData>\n"
+               "  <PPoints>\n"
+               "   <PDataArray type=\"Float64\" NumberOfComponents=\"3\"/>\n"
+               "  </PPoints>\n"
+               "  <Piece Source=\"%s\"/>\n"
+               " </PUnstructuredGrid>\n"
+               "</VTKFile>\n",
+               dof, piece_name.c_str());
+  if (std::fclose(f) != 0) return fail(TBX_EINVAL, "tbx_tree_write_vtu: write to %s failed", pv.c_str());
+  return TBX_OK;
+}
+
+}  // namespace
+
+int tbx_tree_write_vtu(const tbx_tree *tc, const char *prefix, int order, tbx_stream s) {
+  if (!tc || !prefix) return fail(TBX_EINVAL, "tbx_tree_write_vtu: null argument");
+  if (order < 2 || order > TBX_MAX_Q + 1)
+    return fail(TBX_EINVAL, "tbx_tree_write_vtu: order %d outside 2..%d", order, TBX_MAX_Q + 1);
+  tbx_tree *t = const_cast<tbx_tree *>(tc);
+  hipStream_t st = (hipStream_t)s;
+  const int n = order, n3 = n * n * n;
+  const int64_t npts = t->L * (int64_t)n3, ncells = t->L * (int64_t)(n - 1) * (n - 1) * (n - 1);
+  // the leaf's own polynomial on its faces: evaluation points are pulled
+  // inside the leaf by a factor 1 - 2^-40 of the offset from its corner, so
+  // an upper-face point is not located in the neighbour; the file holds the
+  // uniform points
+  double node[TBX_MAX_Q + 1];
+  std::vector<double> unode(n);
+  for (int a = 0; a < n; ++a) {
+    unode[a] = (double)a / (n - 1);
+    node[a] = unode[a] * (1.0 - std::ldexp(1.0, -40));
+  }
+  double *pts = nullptr, *val = nullptr;
+  std::vector<double> hval((size_t)npts * t->dof);
+  auto body = [&]() -> int {
+    HIP_TRY(hipMallocAsync((void **)&pts, sizeof(double) * 3 * (size_t)npts, st));
+    HIP_TRY(hipMallocAsync((void **)&val, sizeof(double) * t->dof * (size_t)npts, st));
+    HIP_TRY(launch_grid_points(t->view(), node, n, pts, st));
+    HIP_TRY(dispatch_eval(t->q, kFma64, t->view(), pts, npts, TBX_BC_FREESPACE, 0, val, st));
+    HIP_TRY(hipMemcpyAsync(hval.data(), val, sizeof(double) * hval.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return TBX_OK;
+  };
+  int rc = body();
+  hipFreeAsync(pts, st);
+  hipFreeAsync(val, st);
+  if (rc) return rc;
+  std::vector<double> xyz((size_t)npts * 3);
+  std::vector<int64_t> conn((size_t)ncells * 8), offs(ncells);
+  std::vector<uint8_t> types(ncells, 12);  // VTK_HEXAHEDRON
+  parallel_for(t->L, [&](int64_t j) {
+    const uint4 inf = t->h_info[j];
+    const double len = 1.0 / (double)(1u << inf.w);
+    const int64_t p0 = j * n3;
+    for (int m = 0; m < n3; ++m) {
+      double *o = &xyz[(size_t)(p0 + m) * 3];
+      o[0] = (double)inf.x * len + len * unode[m % n];
+      o[1] = (double)inf.y * len + len * unode[(m / n) % n];
+      o[2] = (double)inf.z * len + len * unode[m / (n * n)];
+    }
+    const int64_t c0 = j * (int64_t)(n - 1) * (n - 1) * (n - 1);
+    int64_t c = c0;
+    for (int k = 0; k + 1 < n; ++k)
+      for (int b = 0; b + 1 < n; ++b)
+        for (int a = 0; a + 1 < n; ++a, ++c) {
+          auto id = [&](int da, int db, int dk) {
+            return p0 + (a + da) + (int64_t)n * ((b + db) + (int64_t)n * (k + dk));
+          };
+          int64_t *q = &conn[(size_t)c * 8];
+          q[0] = id(0, 0, 0);
+          q[1] = id(1, 0, 0);
+          q[2] = id(1, 1, 0);
+          q[3] = id(0, 1, 0);
+          q[4] = id(0, 0, 1);
+          q[5] = id(1, 0, 1);
+          q[6] = id(1, 1, 1);
+          q[7] = id(0, 1, 1);
+          offs[c] = 8 * (c + 1);
+        }
+  });
+  return write_vtu_files(prefix, t->dof, npts, ncells, xyz, hval, conn, offs, types);
 }
 
 }  // extern "C"

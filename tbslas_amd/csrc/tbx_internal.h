@@ -206,6 +206,36 @@ hipError_t launch_grid_step(int mode, const TreeView &tv, const double *vs, cons
                             int cn, int cdof, const GridStepArgs &a, double *out, double *dep,
                             hipStream_t st);
 
+// The refit (tbx_refit.hip): T[k][a] = T_k(node a mapped to [-1, 1]) for
+// the d = q + 1 stretched nodes.
+struct RefitTable {
+  double T[TBX_MAX_Q + 1][TBX_MAX_Q + 1];
+};
+// C = A H on the matrix cores: A [rows][lda] (K columns used), H [KP][np]
+// zero-padded, C [rows][ldc] (ncols columns written), C32 optional.
+hipError_t launch_refit_gemm(const double *A, int64_t lda, int64_t rows, int K, int KP,
+                             const double *H, int np, double *C, int64_t ldc, int ncols,
+                             float *C32, hipStream_t s);
+// W = S V per row (S = the table's one-axis matrix on each axis, restricted
+// to the triangle): row r = leaf * dof + l reads V[leaf d^3 dof + l cs +
+// p ps]; W [rows][ldw] (packed triangle, zero tail).
+hipError_t launch_refit_project(int d, const double *V, int64_t rows, int dof, int64_t cs,
+                                int64_t ps, const RefitTable &tab, double *W, int ldw,
+                                hipStream_t s);
+// V = B U per row: U [rows][KP] (packed triangle), V [rows][d^3].
+hipError_t launch_refit_expand(int d, const double *U, int64_t rows, const RefitTable &tab,
+                               int KP, double *V, hipStream_t s);
+// CheckChebOutput's block sums (metric_blocks(npts) blocks; pass 0: dof
+// sums per block, pass 1: 4 values per block); dof <= 8.
+int metric_blocks(int64_t npts);
+hipError_t launch_metric(const uint4 *info, int n, const double *val, const double *ref,
+                         int64_t npts, int dof, const double *avg, int pass, double *partial,
+                         hipStream_t s);
+// d = x - y and per-block partial sums of d^2 (diff_norm2_blocks(n) of them).
+int diff_norm2_blocks(int64_t n);
+hipError_t launch_diff_norm2(const double *x, const double *y, int64_t n, double *d,
+                             double *partial, hipStream_t s);
+
 // Degree-independent launchers (tbx_common.hip).
 hipError_t launch_locate(const TreeView &t, const double *pts, int64_t n,
                          int bc, int32_t *leaf, hipStream_t s);

@@ -47,3 +47,11 @@ def check_cheb_output(anchor, depth, q: int, evaluate, fn):
     mx, mx_err = np.abs(ref).max(), np.abs(err).max()
     return (float(np.sqrt(l2_err)), float(np.sqrt(l2_err / l2)), float(mx_err),
             float(mx_err / mx))
+
+
+def check_cheb_output_device(tree, fn, mode: str = "fp64"):
+    """The same metric computed on the device (tbx_check_cheb_output) for a
+    device tree against fn (torch: points [m,3] -> [m,dof] or [m])."""
+    pts = tree.error_points()
+    ref = fn(pts).reshape(pts.shape[0], tree.dof).contiguous()
+    return tree.check_cheb_output(ref, mode)

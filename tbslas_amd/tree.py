@@ -214,6 +214,60 @@ class Tree:
         check(lib().tbx_collect_grid_points(self._h, _ptr(out), _stream()))
         return out
 
+    def first_kind_points(self):
+        """First-kind Chebyshev points of every leaf on the device
+        [L*(q+1)^3][3], where the initial field is sampled for fit()."""
+        import torch
+
+        out = torch.empty((self.L * self.P, 3), dtype=torch.float64, device="cuda")
+        check(lib().tbx_tree_first_kind_points(self._h, _ptr(out), _stream()))
+        return out
+
+    def fit(self, vals):
+        """InitTree's fit on the device: vals [L*(q+1)^3][dof] (point-major)
+        at first_kind_points() become the coefficients."""
+        import torch
+
+        _require_dev(vals, torch.float64, "vals")
+        if vals.numel() != self.L * self.P * self.dof:
+            raise ValueError("vals must hold L*(q+1)^3*dof values")
+        check(lib().tbx_tree_fit(self._h, _ptr(vals), _stream()))
+
+    def fit_function(self, fn):
+        """Fit fn (torch: points [n][3] -> values [n][dof] or [n]) on the
+        device."""
+        pts = self.first_kind_points()
+        v = fn(pts).reshape(pts.shape[0], self.dof).contiguous()
+        self.fit(v)
+
+    def error_points(self):
+        """The CheckChebOutput points [L*n^3][3], n = q/3 + 1."""
+        import torch
+
+        n = self.q // 3 + 1
+        out = torch.empty((self.L * n ** 3, 3), dtype=torch.float64, device="cuda")
+        check(lib().tbx_error_points(self._h, _ptr(out), _stream()))
+        return out
+
+    def check_cheb_output(self, ref, mode: str = "fp64"):
+        """CheckChebOutput on the device against ref [L*n^3][dof] (the exact
+        field at error_points()): (al2, rl2, ali, rli)."""
+        import torch
+
+        _require_dev(ref, torch.float64, "ref")
+        n = self.q // 3 + 1
+        if ref.numel() != self.L * n ** 3 * self.dof:
+            raise ValueError("ref must hold L*n^3*dof values")
+        res = np.zeros(4, dtype=np.float64)
+        check(lib().tbx_check_cheb_output(self._h, _ptr(ref), _mode(mode), _ptr(res),
+                                          _stream()))
+        return tuple(float(v) for v in res)
+
+    def write_vtu(self, prefix: str, order: int = 4) -> None:
+        """Write2File: <prefix>.pvtu and <prefix>_000000.vtu with the tree's
+        values on order^3 uniform points per leaf (VTK hexahedra)."""
+        check(lib().tbx_tree_write_vtu(self._h, prefix.encode(), order, _stream()))
+
     def set_grid_values(self, vals):
         """SetTreeGridValues with vals [L][dof][P] (dof-major per leaf)."""
         import torch
@@ -363,6 +417,14 @@ def pt2coeff_matrix(q: int) -> np.ndarray:
     M = np.empty(((q + 1) ** 3, ncoef(q)), dtype=np.float64)
     check(lib().tbx_pt2coeff_matrix(q, _ptr(M)))
     return M
+
+
+def refit_gram_inverse(q: int) -> np.ndarray:
+    """H = (B^T B)^-1 [Nc][Nc], the factor of pinv(B) = H B^T the device
+    refit applies."""
+    H = np.empty((ncoef(q), ncoef(q)), dtype=np.float64)
+    check(lib().tbx_refit_gram_inverse(q, _ptr(H)))
+    return H
 
 
 def uniform_leaves(depth: int):

@@ -77,3 +77,15 @@ def test_grid_points_layout(orc, tb):
         assert (g[j, :, 0] == c[0] + 0.5 * x[m % 4]).all()
         assert (g[j, :, 1] == c[1] + 0.5 * x[(m // 4) % 4]).all()
         assert (g[j, :, 2] == c[2] + 0.5 * x[m // 16]).all()
+
+
+@pytest.mark.parametrize("q", [1, 2, 4, 8, 10, 14])
+def test_factored_refit_operator(orc, tb, q):
+    # the device refit applies pinv(B) = H B^T: H must invert the Gram
+    # matrix B^T B and H B^T must be the oracle's operator M^T
+    B = basis_matrix(orc, q)
+    H = tb.refit_gram_inverse(q)
+    assert np.abs(H - H.T).max() <= 1e-14 * np.abs(H).max()
+    assert np.abs(H @ (B.T @ B) - np.eye(B.shape[1])).max() <= 1e-12
+    M = orc.pt2coeff(q)
+    assert np.abs(H @ B.T - M.T).max() <= 1e-13 * np.abs(M).max()
