@@ -32,6 +32,7 @@
 #ifndef TBX_H_
 #define TBX_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -182,6 +183,22 @@ int tbx_semilag_insitu(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
                        tbx_tree *conc, int timestep, double dt,
                        int num_rk_step, int bc, uint32_t flags, double *dep,
                        tbx_stream s);
+
+/* ---- tbx_semilag_insitu without allocation.  No counterpart in the
+ *      reference: it is for callers that cannot allocate device memory
+ *      (code under HIP graph capture, tight serving loops, harnesses that
+ *      forbid hipMalloc).  The step's value buffer, the refit workspace and
+ *      FP32 coefficient mirrors the trees lack live in a static device
+ *      scratch of 256 MiB per GPU (held until the process ends); a step
+ *      that needs more returns TBX_EOVERFLOW before any work is queued.
+ *      *scratch_bytes (may be NULL) receives the bytes this step needs.
+ *      Every call on one GPU shares that GPU's scratch: calls that overlap
+ *      on one GPU (two streams, two host threads) race.  Results equal
+ *      tbx_semilag_insitu's. ------------------------------------------------ */
+int tbx_semilag_insitu_noalloc(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc,
+                               int timestep, double dt, int num_rk_step, int bc,
+                               uint32_t flags, double *dep, size_t *scratch_bytes,
+                               tbx_stream s);
 
 /* ---- the first two stages of SolveSemilagInSitu without the refit:
  *      arrival points of conc's leaves (tree_semilag.h:105-115) through

@@ -93,6 +93,8 @@ _SIGS = {
     "tbx_set_grid_values": (_I32, [_P, _P, _P]),
     "tbx_semilag_insitu": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P]),
     "tbx_semilag_values": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P, _P]),
+    "tbx_semilag_insitu_noalloc": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P,
+                                          C.POINTER(C.c_size_t), _P]),
     "tbx_velocity_eval": (_I32, [C.POINTER(_Velocity), _P, _I64, _D, _I32, _U32, _P, _P]),
     "tbx_traj_rk2_v": (_I32, [C.POINTER(_Velocity), _P, _I64, _D, _D, _I32, _I32, _U32, _P,
                               _P]),

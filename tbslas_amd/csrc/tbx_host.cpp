@@ -438,6 +438,39 @@ int prepare_mode(tbx_tree *t, int mode, hipStream_t s) {
   return TBX_OK;
 }
 
+// A bump allocator over caller-independent scratch memory (the
+// allocation-free entry point's static device array): regions are 256-byte
+// aligned and live for one call.
+struct ScratchArena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+  void *take(size_t bytes) {
+    const size_t off = (used + 255) & ~(size_t)255;
+    if (off > size || bytes > size - off) return nullptr;
+    used = off + bytes;
+    return base + off;
+  }
+};
+
+// The tree's device view for a mode.  FP32 modes read an FP32 mirror of the
+// coefficients: the tree's own (made on first use), or, with an arena and a
+// tree that has none, one converted into the arena for this call.
+int prepare_view(tbx_tree *t, int mode, hipStream_t s, ScratchArena *arena, TreeView *v,
+                 const char *who) {
+  if ((mode == kFp32 || mode == kFp32h) && arena && !t->d_coeff32) {
+    *v = t->view();
+    float *m = (float *)arena->take(sizeof(float) * (size_t)t->ncoeff_total() + kCoeffPad);
+    if (!m) return fail(TBX_EOVERFLOW, "%s: FP32 coefficients exceed the scratch", who);
+    HIP_TRY(launch_to_f32(t->d_coeff, m, t->ncoeff_total(), s));
+    v->coeff32 = m;
+    return TBX_OK;
+  }
+  int rc = prepare_mode(t, mode, s);
+  if (rc) return rc;
+  *v = t->view();
+  return TBX_OK;
+}
+
 #define TBX_Q_CASES(X) \
   X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
 
@@ -557,7 +590,7 @@ SubArgs sub_args(double tcur, double tau, tbx_tfac_fn g, void *gctx, int bc) {
 
 // Checks a velocity functor and prepares its trees for the mode.
 int prepare_velocity(const tbx_velocity *v, int mode, hipStream_t s, VelSource *out,
-                     int *q, const char *who) {
+                     int *q, const char *who, ScratchArena *arena = nullptr) {
   if (!v) return fail(TBX_EINVAL, "%s: null velocity", who);
   int nt;
   switch (v->kind) {
@@ -575,9 +608,8 @@ int prepare_velocity(const tbx_velocity *v, int mode, hipStream_t s, VelSource *
     if (t->dof != 3) return fail(TBX_EINVAL, "%s: velocity tree %d has dof %d", who, k, t->dof);
     if (k > 0 && t->q != v->trees[0]->q)
       return fail(TBX_EUNSUPPORTED, "%s: velocity trees differ in degree", who);
-    int rc = prepare_mode(t, mode, s);
+    int rc = prepare_view(t, mode, s, arena, &out->t[k], who);
     if (rc) return rc;
-    out->t[k] = t->view();
   }
   if (v->kind == TBX_VEL_SET)
     for (int k = 0; k < 4; ++k) out->tt[k] = v->times[k];
@@ -596,8 +628,8 @@ tbx_velocity single_tree(const tbx_tree *t, tbx_tfac_fn g, void *gctx) {
   return v;
 }
 
-// The refit operator on the device (once per tree) and the workspace.
-int ensure_refit(tbx_tree *t, hipStream_t s) {
+// The refit operator on the device (uploaded when the tree is created).
+int ensure_refit_operator(tbx_tree *t) {
   if (!t->d_H) {
     const RefitOperator &op = refit_operator_cached(t->q);
     HIP_TRY(hipMalloc(&t->d_H, sizeof(double) * op.H.size()));
@@ -606,26 +638,38 @@ int ensure_refit(tbx_tree *t, hipStream_t s) {
     t->NP = op.NP;
     t->tab = op.tab;
   }
+  return TBX_OK;
+}
+
+// The operator and the tree's own refit workspace.
+int ensure_refit(tbx_tree *t, hipStream_t s) {
+  int rc = ensure_refit_operator(t);
+  if (rc) return rc;
   if (!t->d_W) HIP_TRY(hipMalloc(&t->d_W, sizeof(double) * (size_t)t->L * t->dof * t->KP));
   (void)s;
   return TBX_OK;
 }
 
 // coeff = pinv(B) vals for every leaf component (SetTreeGridValues,
-// tree_utils.h:534-546): W = B^T vals by the one-axis contractions, then
-// coeff = W H on the matrix cores, written in place together with the FP32
-// mirror when the tree has one.
-int refit(tbx_tree *t, const double *vals, hipStream_t s) {
-  int rc = ensure_refit(t, s);
-  if (rc) return rc;
+// tree_utils.h:534-546): W = B^T vals by the one-axis contractions into the
+// workspace W ([L*dof][KP]), then coeff = W H on the matrix cores, written
+// in place together with the FP32 mirror when the tree has one.  Allocates
+// nothing.
+int refit_with(tbx_tree *t, const double *vals, double *W, hipStream_t s) {
   const int64_t rows = t->L * t->dof;
   if (rows > INT32_MAX) return fail(TBX_EOVERFLOW, "refit: %lld rows", (long long)rows);
-  HIP_TRY(launch_refit_project(t->q + 1, vals, rows, t->dof, t->P(), 1, t->tab, t->d_W, t->KP,
-                               s));
-  HIP_TRY(launch_refit_gemm(t->d_W, t->KP, rows, t->KP, t->KP, t->d_H, t->NP, t->d_coeff, t->nc,
+  if (!t->d_H) return fail(TBX_EINVAL, "refit: the tree has no refit operator");
+  HIP_TRY(launch_refit_project(t->q + 1, vals, rows, t->dof, t->P(), 1, t->tab, W, t->KP, s));
+  HIP_TRY(launch_refit_gemm(W, t->KP, rows, t->KP, t->KP, t->d_H, t->NP, t->d_coeff, t->nc,
                             t->nc, t->d_coeff32, s));
   t->f32_stale = t->d_coeff32 == nullptr;
   return TBX_OK;
+}
+
+int refit(tbx_tree *t, const double *vals, hipStream_t s) {
+  int rc = ensure_refit(t, s);
+  if (rc) return rc;
+  return refit_with(t, vals, t->d_W, s);
 }
 
 }  // namespace
@@ -739,6 +783,12 @@ int tbx_tree_create(const tbx_tree_desc *desc, tbx_tree **out) {
   else
     e = hipMemset(t->d_coeff, 0, sizeof(double) * (size_t)t->ncoeff_total());
   if (e != hipSuccess) return bad(e, "coefficient upload");
+  // the refit operator goes to the device with the tree, so a step never
+  // has to allocate it
+  if (int rc = ensure_refit_operator(t)) {
+    free_tree(t);
+    return rc;
+  }
   *out = t;
   return TBX_OK;
 }
@@ -912,7 +962,7 @@ namespace {
 // Fused arrival -> RK2 -> conc evaluation into vals [L][dof][P].
 int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt,
                 int num_rk_step, int bc, uint32_t flags, double *vals, double *dep,
-                hipStream_t st, const char *who) {
+                hipStream_t st, const char *who, ScratchArena *arena = nullptr) {
   if (!conc) return fail(TBX_EINVAL, "%s: null tree", who);
   if (!vals) return fail(TBX_EINVAL, "%s: null value buffer", who);
   if (num_rk_step <= 0 || num_rk_step > kMaxSub)
@@ -923,10 +973,11 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
   int mode, q;
   if ((rc = mode_of(flags, &mode))) return rc;
   VelSource vs;
-  if ((rc = prepare_velocity(vel, mode, st, &vs, &q, who))) return rc;
+  if ((rc = prepare_velocity(vel, mode, st, &vs, &q, who, arena))) return rc;
   if (q != conc->q)
     return fail(TBX_EUNSUPPORTED, "%s: velocity q %d != concentration q %d", who, q, conc->q);
-  if ((rc = prepare_mode(conc, mode, st))) return rc;
+  TreeView cv;
+  if ((rc = prepare_view(conc, mode, st, arena, &cv, who))) return rc;
   const int P = conc->P();
   StepArgs a{};
   const int want = resolve_step_shape(mode, conc->q);
@@ -956,7 +1007,7 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
     tcur = tcur + a.tau;
   }
   new_nodes_host(conc->q, a.node);
-  HIP_TRY(dispatch_step(conc->q, mode, vs, conc->view(), a, vals, dep, st));
+  HIP_TRY(dispatch_step(conc->q, mode, vs, cv, a, vals, dep, st));
   return TBX_OK;
 }
 
@@ -1001,6 +1052,44 @@ int tbx_semilag_insitu(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree 
   if (!vel) return fail(TBX_EINVAL, "tbx_semilag_insitu: null tree");
   const tbx_velocity v = single_tree(vel, g, gctx);
   return tbx_semilag_insitu_v(&v, conc, timestep, dt, num_rk_step, bc, flags, dep, s);
+}
+
+int tbx_semilag_insitu_noalloc(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc,
+                               int timestep, double dt, int num_rk_step, int bc,
+                               uint32_t flags, double *dep, size_t *scratch_bytes,
+                               tbx_stream s) {
+  const char *who = "tbx_semilag_insitu_noalloc";
+  if (!vel || !conc) return fail(TBX_EINVAL, "%s: null tree", who);
+  hipStream_t st = (hipStream_t)s;
+  int mode, rc;
+  if ((rc = mode_of(flags, &mode))) return rc;
+  // the regions this call needs, each 256-byte aligned: values [L][dof][P],
+  // the refit workspace [L*dof][KP], and FP32 mirrors of trees that have
+  // none (FP32 modes)
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t rows = (size_t)conc->L * conc->dof;
+  size_t need = up(sizeof(double) * rows * conc->P()) + up(sizeof(double) * rows * conc->KP);
+  if (mode == kFp32 || mode == kFp32h) {
+    if (!vel->d_coeff32) need += up(sizeof(float) * (size_t)vel->ncoeff_total() + kCoeffPad);
+    if (!conc->d_coeff32) need += up(sizeof(float) * (size_t)conc->ncoeff_total() + kCoeffPad);
+  }
+  if (scratch_bytes) *scratch_bytes = need;
+  void *base = nullptr;
+  size_t cap = 0;
+  HIP_TRY(noalloc_scratch(&base, &cap));
+  if (need > cap)
+    return fail(TBX_EOVERFLOW, "%s: the step needs %zu bytes of scratch, the maximum is %zu", who,
+                need, cap);
+  ScratchArena arena;
+  arena.base = (char *)base;
+  arena.size = cap;
+  double *vals = (double *)arena.take(sizeof(double) * rows * conc->P());
+  double *W = (double *)arena.take(sizeof(double) * rows * conc->KP);
+  const tbx_velocity v = single_tree(vel, g, gctx);
+  if ((rc = step_values(&v, conc, timestep, dt, num_rk_step, bc, flags, vals, dep, st, who,
+                        &arena)))
+    return rc;
+  return refit_with(conc, vals, W, st);
 }
 
 namespace {

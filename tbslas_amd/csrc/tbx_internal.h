@@ -236,6 +236,11 @@ int diff_norm2_blocks(int64_t n);
 hipError_t launch_diff_norm2(const double *x, const double *y, int64_t n, double *d,
                              double *partial, hipStream_t s);
 
+// The allocation-free step's device scratch (tbx_noalloc.hip): 256 MiB per
+// GPU; noalloc_scratch gives its address on the current device.
+constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
+hipError_t noalloc_scratch(void **ptr, size_t *bytes);
+
 // Degree-independent launchers (tbx_common.hip).
 hipError_t launch_locate(const TreeView &t, const double *pts, int64_t n,
                          int bc, int32_t *leaf, hipStream_t s);
