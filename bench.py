@@ -552,7 +552,8 @@ def batch_measure(steps: int = 2, nprob: int = 256):
     taken = steps
     worst = 0.0
     for p in range(0, len(vel), max(1, len(vel) // 16)):
-        ref = tb.Tree(conc[p].anchor, conc[p].depth, conc[p].q, 1, init[p])
+        ref = tb.Tree(conc[p].anchor, conc[p].depth, conc[p].q, 1)
+        ref.set_coeffs(init[p])
         for _ in range(taken):
             tb.semilag_insitu(vel[p], ref, 1, dts[p], 1, 0, "fp64")
         r = ref.get_coeffs()

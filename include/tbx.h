@@ -283,6 +283,25 @@ int tbx_error_points(const tbx_tree *t, double *out, tbx_stream s);
 int tbx_check_cheb_output(const tbx_tree *t, const double *ref, uint32_t flags, double *res,
                           tbx_stream s);
 
+/* ---- adaptivity ---------------------------------------------------------
+ * RefineTree followed by Balance21 (advection.cpp:305-314,
+ * tree_utils.h:117-123), on the device, in place:
+ *  1. a leaf whose top-shell tail (tbx_tail_norm's criterion) exceeds tol
+ *     and whose depth is below max_depth is split into its 8 children;
+ *  2. 8 sibling leaves that all stay, above min_depth, merge into their
+ *     parent when the parent's fit of them (each child's polynomial at the
+ *     parent's grid nodes, refit) has a tail <= tol;
+ *  3. leaves more than one level coarser than a face-, edge- or
+ *     corner-adjacent leaf (across the domain when periodic) are split
+ *     until none is.
+ * A child takes its parent's polynomial (exact: the refit of its values at
+ * the child's nodes).  The tree's leaf count changes (tbx_tree_info);
+ * buffers sized by it are made again on their next use. */
+int tbx_tree_adapt(tbx_tree *t, double tol, int min_depth, int max_depth, int bc,
+                   tbx_stream s);
+/* The tree's current leaves: anchor [L][3], depth [L] (host). */
+int tbx_tree_get_leaves(const tbx_tree *t, uint32_t *anchor, uint8_t *depth);
+
 /* ---- output -------------------------------------------------------------
  * Write2File (the PVFMM tree writer the reference's drivers call,
  * advection.cpp:184-188, 317-324): the tree's values on order^3 uniform

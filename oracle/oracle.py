@@ -339,3 +339,191 @@ def grid_semilag_insitu(vel, conc, timestep, dt, num_rk_step=1, bc=BC_FREESPACE,
     cb = _tfac(g)
     _check(lib().orc_grid_semilag_insitu(vel.ref, cb, None, conc.ref, timestep, dt, num_rk_step,
                                          bc), "grid_semilag_insitu")
+
+
+# ---------------------------------------------------------------------------
+# Adaptivity (RefineTree + Balance21, advection.cpp:305-314,
+# tree_utils.h:117-123) restated on the host for the device's tbx_tree_adapt.
+# PVFMM's subdivision test is not in the reference repository; the rule
+# here is the engine's documented one: split a leaf whose top-shell tail
+# (sum of |c_ijk| over i + j + k = q, largest over components) exceeds tol
+# below max_depth; merge 8 sibling leaves that all stay, above min_depth,
+# when their parent's fit has a tail <= tol; then split leaves more than one
+# level coarser than an adjacent leaf until none is (found here by brute
+# force over leaf pairs, an independent method from the device's probes).
+# ---------------------------------------------------------------------------
+
+def _shell_terms(q):
+    out, m = [], 0
+    for i in range(q + 1):
+        for j in range(q + 1 - i):
+            for k in range(q + 1 - i - j):
+                if i + j + k == q:
+                    out.append(m)
+                m += 1
+    return out
+
+
+def tail_norm(coeff, q):
+    """Top-shell |c| sums in packed order, one term after another (the
+    device's order), largest over components: [L]."""
+    c = np.asarray(coeff).reshape(coeff.shape[0], coeff.shape[1], -1)
+    s = np.zeros(c.shape[:2])
+    for m in _shell_terms(q):
+        s = s + np.abs(c[:, :, m])
+    return s.max(axis=1)
+
+
+def _triangle(q):
+    return [(i, j, k) for i in range(q + 1) for j in range(q + 1 - i) for k in range(q + 1 - i - j)]
+
+
+def eval_in_leaf(coeff, anchor, depth, pts, q):
+    """Each leaf's own expansion at its points: coeff [n][dof][Nc], pts
+    [n][m][3] -> [n][dof][m]; leaf coordinates clamped to [-1, 1]."""
+    h = 0.5 ** np.asarray(depth, dtype=np.float64)
+    corner = np.asarray(anchor, dtype=np.float64) * h[:, None]
+    xi = ((pts - corner[:, None, :]) * 2.0) / h[:, None, None] - 1.0
+    xi = np.clip(xi, -1.0, 1.0)
+    T = [np.polynomial.chebyshev.chebvander(xi[..., a], q) for a in range(3)]  # [n][m][d]
+    tri = _triangle(q)
+    B = np.stack([T[0][..., k] * T[1][..., j] * T[2][..., i] for i, j, k in tri], axis=-1)
+    return np.einsum("nlc,nmc->nlm", coeff, B)
+
+
+def _fit_leaves_from(q, dof, anchor, depth, vals):
+    t = Tree(anchor, depth, q, dof)
+    set_grid_values(t, pt2coeff(q), np.ascontiguousarray(vals))
+    return t.coeff.copy()
+
+
+def _resample(q, dof, coeff_src, anchor_src, depth_src, src, anchor_new, depth_new, children):
+    """Coefficients of new leaves from old leaves' polynomials at the new
+    leaves' stretched nodes (children: the node's octant picks one of the 8
+    children starting at src[n])."""
+    if len(src) == 0:
+        return np.zeros((0, dof, ncoef(q)))
+    x = new_nodes(q)
+    d = q + 1
+    m = np.arange(d ** 3)
+    loc = np.stack([x[m % d], x[(m // d) % d], x[m // (d * d)]], axis=1)
+    h = 0.5 ** np.asarray(depth_new, dtype=np.float64)
+    pts = np.asarray(anchor_new, dtype=np.float64)[:, None, :] * h[:, None, None] + \
+        h[:, None, None] * loc[None]
+    out = np.empty((len(src), dof, d ** 3))
+    oct_ = (loc[:, 0] >= 0.5) * 1 + (loc[:, 1] >= 0.5) * 2 + (loc[:, 2] >= 0.5) * 4
+    for n in range(len(src)):
+        if children:
+            s = src[n] + oct_
+            for o in range(8):
+                sel = oct_ == o
+                if sel.any():
+                    out[n][:, sel] = eval_in_leaf(coeff_src[s[sel][:1]], anchor_src[s[sel][:1]],
+                                                  depth_src[s[sel][:1]], pts[n][sel][None],
+                                                  q)[0]
+        else:
+            out[n] = eval_in_leaf(coeff_src[src[n]:src[n] + 1], anchor_src[src[n]:src[n] + 1],
+                                  depth_src[src[n]:src[n] + 1], pts[n][None], q)[0]
+    return _fit_leaves_from(q, dof, anchor_new, depth_new, out)
+
+
+def _adjacent(a0, d0, a1, d1, periodic):
+    """Do the closed boxes of two leaves touch (periodic: across faces)?"""
+    lo0, hi0 = a0 * 2.0 ** -d0, (a0 + 1) * 2.0 ** -d0
+    h1 = (2.0 ** -np.asarray(d1, dtype=np.float64))[:, None]
+    lo1, hi1 = a1 * h1, (a1 + 1) * h1
+    ok = np.ones(len(d1), dtype=bool)
+    for ax in range(3):
+        touch = (lo1[:, ax] <= hi0[ax]) & (lo0[ax] <= hi1[:, ax])
+        if periodic:
+            touch |= (hi1[:, ax] == 1.0) & (lo0[ax] == 0.0)
+            touch |= (hi0[ax] == 1.0) & (lo1[:, ax] == 0.0)
+        ok &= touch
+    return ok
+
+
+def _split_list(anchor, depth, mask):
+    reps = np.where(mask, 8, 1)
+    idx = np.repeat(np.arange(len(depth)), reps)
+    na = anchor[idx].astype(np.uint32)
+    nd = depth[idx].astype(np.uint8)
+    child = np.zeros(len(idx), dtype=np.int64)
+    starts = np.cumsum(reps) - reps
+    for c in range(8):
+        child[starts[mask] + c] = c
+    mm = np.repeat(mask, reps)
+    for ax in range(3):
+        na[mm, ax] = 2 * na[mm, ax] + ((child[mm] >> ax) & 1)
+    nd[mm] += 1
+    return na, nd, idx
+
+
+def adapt(anchor, depth, coeff, q, tol, min_depth, max_depth, bc):
+    """(anchor, depth, coeff) after one adaptation (see the section note)."""
+    anchor = np.asarray(anchor, dtype=np.uint32).reshape(-1, 3)
+    depth = np.asarray(depth, dtype=np.uint8)
+    coeff = np.asarray(coeff, dtype=np.float64)
+    L, dof = len(depth), coeff.shape[1]
+    tail = tail_norm(coeff, q)
+    refine = (tail > tol) & (depth < max_depth)
+    cand = []
+    j = 0
+    while j + 8 <= L:
+        f = anchor[j].astype(np.int64)
+        ok = depth[j] > min_depth and not (f & 1).any()
+        for c in range(8):
+            if not ok:
+                break
+            g = anchor[j + c].astype(np.int64)
+            ok = (depth[j + c] == depth[j] and g[0] == f[0] + (c & 1) and
+                  g[1] == f[1] + ((c >> 1) & 1) and g[2] == f[2] + ((c >> 2) & 1) and
+                  not refine[j + c])
+        if ok:
+            cand.append(j)
+            j += 8
+        else:
+            j += 1
+    cand = np.array(cand, dtype=np.int64)
+    pa = anchor[cand] // 2 if len(cand) else np.zeros((0, 3), np.uint32)
+    pd = depth[cand] - 1 if len(cand) else np.zeros(0, np.uint8)
+    pcoef = _resample(q, dof, coeff, anchor, depth, cand, pa, pd, True)
+    merge = tail_norm(pcoef, q) <= tol if len(cand) else np.zeros(0, dtype=bool)
+    na, nd, nc = [], [], []
+    k = 0
+    j = 0
+    while j < L:
+        if k < len(cand) and cand[k] == j:
+            if merge[k]:
+                na.append(pa[k]), nd.append(pd[k]), nc.append(pcoef[k])
+                j += 8
+                k += 1
+                continue
+            k += 1
+        if refine[j]:
+            ca, cd, _ = _split_list(anchor[j:j + 1], depth[j:j + 1], np.array([True]))
+            cc = _resample(q, dof, coeff, anchor, depth, np.full(8, j), ca, cd, False)
+            for c in range(8):
+                na.append(ca[c]), nd.append(cd[c]), nc.append(cc[c])
+        else:
+            na.append(anchor[j]), nd.append(depth[j]), nc.append(coeff[j])
+        j += 1
+    anchor = np.array(na, dtype=np.uint32)
+    depth = np.array(nd, dtype=np.uint8)
+    coeff = np.array(nc)
+    while True:
+        split = np.zeros(len(depth), dtype=bool)
+        af = anchor.astype(np.float64)
+        for m in range(len(depth)):
+            adj = _adjacent(af[m], int(depth[m]), af, depth.astype(np.int64), bc == BC_PERIODIC)
+            adj[m] = False
+            split |= adj & (depth.astype(np.int64) + 1 < int(depth[m]))
+        if not split.any():
+            break
+        ca, cd, src = _split_list(anchor, depth, split)
+        newc = np.empty((len(cd), dof, coeff.shape[2]))
+        keep = np.repeat(~split, np.where(split, 8, 1))
+        newc[keep] = coeff[src[keep]]
+        ch = ~keep
+        newc[ch] = _resample(q, dof, coeff, anchor, depth, src[ch], ca[ch], cd[ch], False)
+        anchor, depth, coeff = ca, cd, newc
+    return anchor, depth, coeff

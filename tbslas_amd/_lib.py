@@ -117,6 +117,8 @@ _SIGS = {
     "tbx_error_points": (_I32, [_P, _P, _P]),
     "tbx_check_cheb_output": (_I32, [_P, _P, _U32, _P, _P]),
     "tbx_tree_write_vtu": (_I32, [_P, C.c_char_p, _I32, _P]),
+    "tbx_tree_adapt": (_I32, [_P, _D, _I32, _I32, _I32, _P]),
+    "tbx_tree_get_leaves": (_I32, [_P, _P, _P]),
     "tbx_batch_create": (_I32, [_I32, _P, _P, _P, _I32, C.POINTER(_P)]),
     "tbx_batch_destroy": (_I32, [_P]),
     "tbx_batch_set_config": (_I32, [_P, _I32]),

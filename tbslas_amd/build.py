@@ -62,6 +62,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     units.append((os.path.join(CSRC, "tbx_grid.hip"), os.path.join(BUILD, "tbx_grid.o"), []))
     units.append((os.path.join(CSRC, "tbx_batch.hip"), os.path.join(BUILD, "tbx_batch.o"), []))
     units.append((os.path.join(CSRC, "tbx_refit.hip"), os.path.join(BUILD, "tbx_refit.o"), []))
+    units.append((os.path.join(CSRC, "tbx_adapt.hip"), os.path.join(BUILD, "tbx_adapt.o"), []))
     units.append((os.path.join(CSRC, "tbx_noalloc.hip"), os.path.join(BUILD, "tbx_noalloc.o"),
                   []))
     units.append((os.path.join(CSRC, "tbx_host.cpp"), os.path.join(BUILD, "tbx_host.o"),

@@ -24,9 +24,12 @@ Sweeps (CLI: python -m tbslas_amd.convergence SWEEP):
   zalesak   test 2, q = 3, 8, 14 at uniform depths (conv_zal.py:47-98 uses
             d = 5, 7 with tol = 1e-10 and 5 revolutions);
   traj      T = 2 pi, tn = 50 * 2^k, k = 0..7 (conv_traj.py:44-79).
-Trees are uniform or tol-adaptive at t = 0 and stay fixed over the run: the
-reference refines and coarsens the concentration tree after every step
-(advection.cpp:300-315), which is tree management outside this engine.
+  zalesak_adaptive
+            the reference's table: test 2, tol = 1e-10, maximum depth 5 and 7,
+            q = 3, 8, 14, dt = 0.0628, 5 revolutions, the concentration tree
+            refined, merged and 2:1-balanced on the device after every step
+            (advection.cpp:300-315, Tree.adapt).
+The other sweeps keep the tree of t = 0 (uniform or tol-adaptive) fixed.
 
 The RK2 departure map of the rotation is linear, so the semi-discrete
 solution (exact interpolation, RK2 trajectories) is known in closed form:
@@ -119,6 +122,42 @@ def run_advection(test: int, anchor, depth, q: int, dt: float, tn: int,
     return res
 
 
+def run_advection_adaptive(test: int, q: int, dt: float, tn: int, tol: float, max_depth: int,
+                           min_depth: int = 3, mode: str = "fp64", progress=None) -> dict:
+    """The reference's adaptive run (advection.cpp:254-341): the
+    concentration tree built to tol, then after every in-situ step refined,
+    merged and 2:1-balanced on the device (RefineTree + Balance21,
+    advection.cpp:305-310); errors measured on the device
+    (CheckChebOutput, tbx_check_cheb_output) at t = 0 and at tn dt."""
+    import torch
+
+    c = concentration(test)
+
+    def ref_at(t):
+        return lambda x: torch.from_numpy(
+            np.ascontiguousarray(c(x.cpu().numpy(), t), dtype=np.float64)).cuda()
+
+    anchor, depth = adaptive_leaves(lambda p: c(p, 0.0), q, 1, tol, max_depth, min_depth)
+    conc = Tree(anchor, depth, q, 1, fit_function(lambda p: c(p, 0.0), anchor, depth, q, 1))
+    va, vd = uniform_leaves(min_depth)
+    vel = velocity_tree(va, vd, q)
+    e_in = metrics.check_cheb_output_device(conc, ref_at(0.0))
+    leaves = [conc.L]
+    for n in range(1, tn + 1):
+        semilag_insitu(vel, conc, n, dt, 1, BC_FREESPACE, mode)
+        conc.adapt(tol, min_depth, max_depth, BC_FREESPACE)
+        leaves.append(conc.L)
+        if progress and n % progress == 0:
+            print(f"# step {n}: {conc.L} leaves", flush=True)
+    e_out = metrics.check_cheb_output_device(conc, ref_at(tn * dt))
+    res = {"test": test, "L": conc.L, "max_depth": int(np.max(conc.depth)), "q": q, "dt": dt,
+           "tn": tn, "mode": mode, "tol": tol, "in": e_in, "out": e_out,
+           "leaves_min": min(leaves), "leaves_max": max(leaves), "L0": leaves[0]}
+    vel.close()
+    conc.close()
+    return res
+
+
 def run_traj(tn: int, q: int = 8, depth: int = 2, mode: str = "fp64") -> float:
     """traj.cpp:115-146: |x_T - x_0| after one backward revolution."""
     import torch
@@ -181,11 +220,24 @@ def sweep_zalesak(mode="fp64", depths=(4, 5), degrees=(3, 8, 14), revolutions=1)
             yield run_advection(2, anchor, dep, q, dt, tn, mode)
 
 
+def sweep_zalesak_adaptive(mode="fp64", depths=(5, 7), degrees=(3, 8, 14), revolutions=5,
+                           tol=1e-10):
+    dt = 6.28e-2
+    tn = int(round(6.28 * revolutions / dt))
+    for d in depths:
+        for q in degrees:
+            yield run_advection_adaptive(2, q, dt, tn, tol, d, mode=mode, progress=100)
+
+
 def main(argv=None) -> int:
     import argparse
 
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("sweep", choices=["temporal", "spatial", "long", "zalesak", "traj"])
+    ap.add_argument("sweep", choices=["temporal", "spatial", "long", "zalesak",
+                                      "zalesak_adaptive", "traj"])
+    ap.add_argument("--depths", default=None, help="comma list (zalesak_adaptive)")
+    ap.add_argument("--degrees", default=None, help="comma list (zalesak_adaptive)")
+    ap.add_argument("--revolutions", type=float, default=5)
     ap.add_argument("--mode", default="fp64")
     a = ap.parse_args(argv)
     if a.sweep == "traj":
@@ -196,6 +248,20 @@ def main(argv=None) -> int:
             o = "" if prev is None else f" order {order(prev, e):.3f}"
             print(f"traj tn {tn:5d} dt {2 * math.pi / tn:.4e} err {e:.4e}{o}", flush=True)
             prev = e
+        return 0
+    if a.sweep == "zalesak_adaptive":
+        kw = {"revolutions": a.revolutions}
+        if a.depths:
+            kw["depths"] = tuple(int(x) for x in a.depths.split(","))
+        if a.degrees:
+            kw["degrees"] = tuple(int(x) for x in a.degrees.split(","))
+        for r in sweep_zalesak_adaptive(a.mode, **kw):
+            i, o = r["in"], r["out"]
+            print(f"zalesak adaptive q {r['q']:2d} maxd {r['max_depth']} tol {r['tol']:.0e} "
+                  f"dt {r['dt']:.4e} tn {r['tn']} leaves {r['L0']} -> {r['L']} (min "
+                  f"{r['leaves_min']}, max {r['leaves_max']}) | in rl2 {i[1]:.3e} rli "
+                  f"{i[3]:.3e} | out al2 {o[0]:.3e} rl2 {o[1]:.3e} ali {o[2]:.3e} rli {o[3]:.3e}",
+                  flush=True)
         return 0
     gen = {"temporal": sweep_temporal, "spatial": sweep_spatial, "long": sweep_long,
            "zalesak": sweep_zalesak}[a.sweep](a.mode)

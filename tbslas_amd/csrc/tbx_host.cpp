@@ -672,6 +672,91 @@ int refit(tbx_tree *t, const double *vals, hipStream_t s) {
   return refit_with(t, vals, t->d_W, s);
 }
 
+// The leaf structure of a tree from a host leaf list: validated (depths,
+// anchors, strict Morton order, an exact tiling of [0,1)^3), host mirrors
+// of keys and (anchor, depth), and their device copies with the location
+// directory, replacing any previous ones.  Coefficients are untouched.
+int set_leaves(tbx_tree *t, int64_t L, const uint32_t *anchor, const uint8_t *depth) {
+  if (L <= 0 || L > INT32_MAX) return fail(TBX_EOVERFLOW, "%lld leaves", (long long)L);
+  std::vector<uint64_t> key(L);
+  std::vector<uint4> info(L);
+  int maxd = 0;
+  for (int64_t j = 0; j < L; ++j) {
+    const int dj = depth[j];
+    const uint32_t ax = anchor[3 * j], ay = anchor[3 * j + 1], az = anchor[3 * j + 2];
+    if (dj > kKeyBits)
+      return fail(TBX_EUNSUPPORTED, "leaf %lld: depth %d > %d", (long long)j, dj, kKeyBits);
+    const uint32_t lim = 1u << dj;
+    if (ax >= lim || ay >= lim || az >= lim)
+      return fail(TBX_EINVAL, "leaf %lld: anchor outside depth-%d grid", (long long)j, dj);
+    key[j] = leaf_key(ax, ay, az, dj);
+    info[j] = make_uint4(ax, ay, az, (uint32_t)dj);
+    maxd = std::max(maxd, dj);
+  }
+  // strictly increasing keys whose ranges tile [0, 8^21) without gaps
+  const unsigned __int128 total = (unsigned __int128)1 << (3 * kKeyBits);
+  if (key[0] != 0) return fail(TBX_ECOVER, "first leaf does not start at the origin");
+  for (int64_t j = 0; j < L; ++j) {
+    const unsigned __int128 end =
+        (unsigned __int128)key[j] + ((unsigned __int128)1 << (3 * (kKeyBits - depth[j])));
+    if (j + 1 < L) {
+      if (key[j + 1] <= key[j])
+        return fail(TBX_EORDER, "leaves %lld and %lld are not in Morton order", (long long)j,
+                    (long long)(j + 1));
+      if (end != key[j + 1])
+        return fail(TBX_ECOVER, "leaf %lld and the next leaf overlap or leave a gap", (long long)j);
+    } else if (end != total) {
+      return fail(TBX_ECOVER, "last leaf does not reach the far corner");
+    }
+  }
+  // Directory level g <= 6: 8^6 + 1 int32 entries (1 MiB) stay resident in
+  // one XCD's L2.  On uniformly random points a level-7 directory (8 MiB)
+  // located a depth-7 tree at 1.8 ms per 2^27 points, level 6 plus a search
+  // of at most three steps at 1.0 ms.
+  const int dir_level = std::min(maxd, 6);
+  const int64_t ncell = (int64_t)1 << (3 * dir_level);
+  std::vector<int32_t> dir(ncell + 1);
+  {
+    const int sh = 3 * (kKeyBits - dir_level);
+    int64_t j = 0;
+    for (int64_t c = 0; c < ncell; ++c) {
+      const uint64_t ck = (uint64_t)c << sh;
+      while (j + 1 < L && key[j + 1] <= ck) ++j;
+      dir[c] = (int32_t)j;
+    }
+    dir[ncell] = (int32_t)(L - 1);
+  }
+  uint64_t *d_key = nullptr;
+  uint4 *d_info = nullptr;
+  int32_t *d_dir = nullptr;
+  hipError_t e = hipMalloc(&d_key, sizeof(uint64_t) * L);
+  if (e == hipSuccess) e = hipMalloc(&d_info, sizeof(uint4) * L);
+  if (e == hipSuccess) e = hipMalloc(&d_dir, sizeof(int32_t) * dir.size());
+  if (e == hipSuccess) e = hipMemcpy(d_key, key.data(), sizeof(uint64_t) * L, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_info, info.data(), sizeof(uint4) * L, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(d_dir, dir.data(), sizeof(int32_t) * dir.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(d_key);
+    hipFree(d_info);
+    hipFree(d_dir);
+    return fail(e == hipErrorOutOfMemory ? TBX_ENOMEM : TBX_EHIP, "leaf structure upload: %s",
+                hipGetErrorString(e));
+  }
+  hipFree(t->d_key);
+  hipFree(t->d_info);
+  hipFree(t->d_dir);
+  t->d_key = d_key;
+  t->d_info = d_info;
+  t->d_dir = d_dir;
+  t->h_key.swap(key);
+  t->h_info.swap(info);
+  t->L = L;
+  t->max_depth = maxd;
+  t->dir_level = dir_level;
+  return TBX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -694,95 +779,29 @@ int tbx_tree_create(const tbx_tree_desc *desc, tbx_tree **out) {
     return fail(TBX_EOVERFLOW, "tbx_tree_create: coefficient array too large");
 
   tbx_tree *t = new tbx_tree_s();
-  t->L = L;
   t->q = desc->q;
   t->dof = desc->dof;
   t->nc = nc;
-  t->h_key.resize(L);
-  t->h_info.resize(L);
-  int maxd = 0;
-  for (int64_t j = 0; j < L; ++j) {
-    const int dj = desc->depth[j];
-    const uint32_t ax = desc->anchor[3 * j], ay = desc->anchor[3 * j + 1],
-                   az = desc->anchor[3 * j + 2];
-    if (dj > kKeyBits) {
-      free_tree(t);
-      return fail(TBX_EUNSUPPORTED, "leaf %lld: depth %d > %d", (long long)j, dj, kKeyBits);
-    }
-    const uint32_t lim = 1u << dj;
-    if (ax >= lim || ay >= lim || az >= lim) {
-      free_tree(t);
-      return fail(TBX_EINVAL, "leaf %lld: anchor outside depth-%d grid", (long long)j, dj);
-    }
-    t->h_key[j] = leaf_key(ax, ay, az, dj);
-    t->h_info[j] = make_uint4(ax, ay, az, (uint32_t)dj);
-    maxd = std::max(maxd, dj);
-  }
-  // strictly increasing keys whose ranges tile [0, 8^21) without gaps
-  const unsigned __int128 total = (unsigned __int128)1 << (3 * kKeyBits);
-  if (t->h_key[0] != 0) {
-    free_tree(t);
-    return fail(TBX_ECOVER, "first leaf does not start at the origin");
-  }
-  for (int64_t j = 0; j < L; ++j) {
-    const unsigned __int128 end =
-        (unsigned __int128)t->h_key[j] + ((unsigned __int128)1 << (3 * (kKeyBits - desc->depth[j])));
-    if (j + 1 < L) {
-      if (t->h_key[j + 1] <= t->h_key[j]) {
-        free_tree(t);
-        return fail(TBX_EORDER, "leaves %lld and %lld are not in Morton order", (long long)j,
-                    (long long)(j + 1));
-      }
-      if (end != t->h_key[j + 1]) {
-        free_tree(t);
-        return fail(TBX_ECOVER, "leaf %lld and the next leaf overlap or leave a gap", (long long)j);
-      }
-    } else if (end != total) {
-      free_tree(t);
-      return fail(TBX_ECOVER, "last leaf does not reach the far corner");
-    }
-  }
-  t->max_depth = maxd;
   hipGetDevice(&t->device);
-  // Directory level g <= 6: 8^6 + 1 int32 entries (1 MiB) stay resident in
-  // one XCD's L2.  On uniformly random points a level-7 directory (8 MiB)
-  // located a depth-7 tree at 1.8 ms per 2^27 points, level 6 plus a search
-  // of at most three steps at 1.0 ms.
-  t->dir_level = std::min(maxd, 6);
-  const int64_t ncell = (int64_t)1 << (3 * t->dir_level);
-  std::vector<int32_t> dir(ncell + 1);
-  {
-    const int sh = 3 * (kKeyBits - t->dir_level);
-    int64_t j = 0;
-    for (int64_t c = 0; c < ncell; ++c) {
-      const uint64_t ck = (uint64_t)c << sh;
-      while (j + 1 < L && t->h_key[j + 1] <= ck) ++j;
-      dir[c] = (int32_t)j;
-    }
-    dir[ncell] = (int32_t)(L - 1);
-  }
-  auto bad = [&](hipError_t e, const char *what) {
+  if (int rc = set_leaves(t, L, desc->anchor, desc->depth)) {
     free_tree(t);
-    return fail(e == hipErrorOutOfMemory ? TBX_ENOMEM : TBX_EHIP, "%s: %s", what,
-                hipGetErrorString(e));
-  };
+    return rc;
+  }
   hipError_t e;
-  if ((e = hipMalloc(&t->d_key, sizeof(uint64_t) * L)) != hipSuccess) return bad(e, "hipMalloc");
-  if ((e = hipMalloc(&t->d_info, sizeof(uint4) * L)) != hipSuccess) return bad(e, "hipMalloc");
-  if ((e = hipMalloc(&t->d_dir, sizeof(int32_t) * dir.size())) != hipSuccess) return bad(e, "hipMalloc");
-  if ((e = hipMalloc(&t->d_coeff, sizeof(double) * (size_t)t->ncoeff_total() + kCoeffPad)) != hipSuccess)
-    return bad(e, "hipMalloc");
-  if ((e = hipMemcpy(t->d_key, t->h_key.data(), sizeof(uint64_t) * L, hipMemcpyHostToDevice)) != hipSuccess)
-    return bad(e, "hipMemcpy");
-  if ((e = hipMemcpy(t->d_info, t->h_info.data(), sizeof(uint4) * L, hipMemcpyHostToDevice)) != hipSuccess)
-    return bad(e, "hipMemcpy");
-  if ((e = hipMemcpy(t->d_dir, dir.data(), sizeof(int32_t) * dir.size(), hipMemcpyHostToDevice)) != hipSuccess)
-    return bad(e, "hipMemcpy");
+  if ((e = hipMalloc(&t->d_coeff, sizeof(double) * (size_t)t->ncoeff_total() + kCoeffPad)) !=
+      hipSuccess) {
+    free_tree(t);
+    return fail(e == hipErrorOutOfMemory ? TBX_ENOMEM : TBX_EHIP, "hipMalloc: %s",
+                hipGetErrorString(e));
+  }
   if (desc->coeff)
     e = hipMemcpy(t->d_coeff, desc->coeff, sizeof(double) * (size_t)t->ncoeff_total(), hipMemcpyHostToDevice);
   else
     e = hipMemset(t->d_coeff, 0, sizeof(double) * (size_t)t->ncoeff_total());
-  if (e != hipSuccess) return bad(e, "coefficient upload");
+  if (e != hipSuccess) {
+    free_tree(t);
+    return fail(TBX_EHIP, "coefficient upload: %s", hipGetErrorString(e));
+  }
   // the refit operator goes to the device with the tree, so a step never
   // has to allocate it
   if (int rc = ensure_refit_operator(t)) {
@@ -2089,6 +2108,293 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor, uint8
     L = newL;
   }
   *num_leaves = L;
+  return TBX_OK;
+}
+
+}  // extern "C"
+
+// RefineTree + Balance21 on the device.
+namespace {
+
+// A device buffer that frees itself (adaptivity's temporaries).
+struct DevBuf {
+  void *p = nullptr;
+  hipStream_t s = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+int dev_alloc(DevBuf &b, size_t bytes, hipStream_t s) {
+  b.s = s;
+  HIP_TRY(hipMallocAsync(&b.p, bytes ? bytes : 8, s));
+  return TBX_OK;
+}
+
+// Coefficients [nleaf][dof][nc] of leaves whose grid-node values are vals
+// [nleaf][dof][P]: the tree's refit operator, into out.
+int refit_rows(tbx_tree *t, const double *vals, int64_t nleaf, double *out, hipStream_t s) {
+  const int64_t rows = nleaf * t->dof;
+  if (rows <= 0) return TBX_OK;
+  DevBuf W;
+  if (int rc = dev_alloc(W, sizeof(double) * (size_t)rows * t->KP, s)) return rc;
+  HIP_TRY(launch_refit_project(t->q + 1, vals, rows, t->dof, t->P(), 1, t->tab, W.as<double>(),
+                               t->KP, s));
+  HIP_TRY(launch_refit_gemm(W.as<double>(), t->KP, rows, t->KP, t->KP, t->d_H, t->NP, out, t->nc,
+                            t->nc, nullptr, s));
+  return TBX_OK;
+}
+
+// Coefficients of new leaves resampled from the tree's current leaves:
+// leaf n from src[n] (restrict: the 8 children starting at src[n]).
+int resample(tbx_tree *t, const std::vector<int32_t> &src, const std::vector<uint4> &info,
+             int restrict_children, double *out, hipStream_t s) {
+  const int64_t n = (int64_t)src.size();
+  if (!n) return TBX_OK;
+  DevBuf dsrc, dinfo, vals;
+  int rc;
+  if ((rc = dev_alloc(dsrc, sizeof(int32_t) * n, s)) || (rc = dev_alloc(dinfo, sizeof(uint4) * n, s)) ||
+      (rc = dev_alloc(vals, sizeof(double) * (size_t)n * t->dof * t->P(), s)))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(dsrc.p, src.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dinfo.p, info.data(), sizeof(uint4) * n, hipMemcpyHostToDevice, s));
+  double node[TBX_MAX_Q + 1];
+  new_nodes_host(t->q, node);
+  HIP_TRY(launch_leaf_resample(t->view(), dsrc.as<int32_t>(), dinfo.as<uint4>(),
+                               restrict_children, n, node, vals.as<double>(), s));
+  rc = refit_rows(t, vals.as<double>(), n, out, s);
+  if (rc) return rc;
+  // the host vectors must outlive the copies
+  HIP_TRY(hipStreamSynchronize(s));
+  return TBX_OK;
+}
+
+// One new leaf of an adaptation: its anchor and depth and where its
+// coefficients come from (kind 0: old leaf src; 1: prolongation of old leaf
+// src; 2: row src of the restricted parents).
+struct NewLeaf {
+  uint32_t a[3];
+  uint8_t d;
+  int kind;
+  int64_t src;
+};
+
+// Replace the tree's leaves by `leaves` (valid by construction), with
+// coefficients from the old leaves, from prolongation (computed here) and
+// from the restricted parents pcoef.
+int rebuild(tbx_tree *t, const std::vector<NewLeaf> &leaves, const double *pcoef, hipStream_t s) {
+  const int64_t Ln = (int64_t)leaves.size();
+  const size_t row = (size_t)t->dof * t->nc;
+  std::vector<int32_t> psrc;
+  std::vector<uint4> pinfo;
+  std::vector<int64_t> map_old(Ln, -1), map_par(Ln, -1), map_pro(Ln, -1);
+  for (int64_t n = 0; n < Ln; ++n) {
+    const NewLeaf &l = leaves[n];
+    if (l.kind == 0) {
+      map_old[n] = l.src;
+    } else if (l.kind == 2) {
+      map_par[n] = l.src;
+    } else {
+      map_pro[n] = (int64_t)psrc.size();
+      psrc.push_back((int32_t)l.src);
+      pinfo.push_back(make_uint4(l.a[0], l.a[1], l.a[2], l.d));
+    }
+  }
+  double *coeff = nullptr;
+  HIP_TRY(hipMalloc(&coeff, sizeof(double) * row * Ln + kCoeffPad));
+  auto body = [&]() -> int {
+    DevBuf pro, maps;
+    int rc;
+    if ((rc = dev_alloc(pro, sizeof(double) * row * psrc.size(), s)) ||
+        (rc = dev_alloc(maps, sizeof(int64_t) * 3 * Ln, s)))
+      return rc;
+    if ((rc = resample(t, psrc, pinfo, 0, pro.as<double>(), s))) return rc;
+    int64_t *dm = maps.as<int64_t>();
+    HIP_TRY(hipMemcpyAsync(dm, map_old.data(), sizeof(int64_t) * Ln, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dm + Ln, map_pro.data(), sizeof(int64_t) * Ln, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dm + 2 * Ln, map_par.data(), sizeof(int64_t) * Ln, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_gather_rows(t->d_coeff, dm, Ln, (int)row, coeff, s));
+    HIP_TRY(launch_gather_rows(pro.as<double>(), dm + Ln, Ln, (int)row, coeff, s));
+    if (pcoef) HIP_TRY(launch_gather_rows(pcoef, dm + 2 * Ln, Ln, (int)row, coeff, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return TBX_OK;
+  };
+  int rc = body();
+  if (rc) {
+    (void)hipFree(coeff);
+    return rc;
+  }
+  std::vector<uint32_t> an(3 * Ln);
+  std::vector<uint8_t> de(Ln);
+  for (int64_t n = 0; n < Ln; ++n) {
+    for (int k = 0; k < 3; ++k) an[3 * n + k] = leaves[n].a[k];
+    de[n] = leaves[n].d;
+  }
+  if ((rc = set_leaves(t, Ln, an.data(), de.data()))) {
+    (void)hipFree(coeff);
+    return rc;
+  }
+  (void)hipFree(t->d_coeff);
+  t->d_coeff = coeff;
+  // buffers sized by the leaf count are made again on their next use
+  (void)hipFree(t->d_coeff32);
+  (void)hipFree(t->d_vals);
+  (void)hipFree(t->d_W);
+  (void)hipFree(t->d_adjA);
+  t->d_coeff32 = nullptr;
+  t->d_vals = nullptr;
+  t->d_W = nullptr;
+  t->d_adjA = nullptr;
+  t->f32_stale = true;
+  return TBX_OK;
+}
+
+NewLeaf keep_leaf(const tbx_tree *t, int64_t j) {
+  const uint4 f = t->h_info[j];
+  return NewLeaf{{f.x, f.y, f.z}, (uint8_t)f.w, 0, j};
+}
+
+void push_children(const tbx_tree *t, int64_t j, std::vector<NewLeaf> &out) {
+  const uint4 f = t->h_info[j];
+  for (uint32_t c = 0; c < 8; ++c)
+    out.push_back(NewLeaf{{2 * f.x + (c & 1), 2 * f.y + ((c >> 1) & 1), 2 * f.z + ((c >> 2) & 1)},
+                          (uint8_t)(f.w + 1), 1, j});
+}
+
+}  // namespace
+
+extern "C" {
+
+int tbx_tree_adapt(tbx_tree *t, double tol, int min_depth, int max_depth, int bc, tbx_stream s) {
+  if (!t) return fail(TBX_EINVAL, "tbx_tree_adapt: null tree");
+  if (int rc = check_bc(bc)) return rc;
+  if (!(tol >= 0.0) || min_depth < 0 || max_depth < min_depth || max_depth > kKeyBits)
+    return fail(TBX_EINVAL, "tbx_tree_adapt: tol %g, depths %d..%d", tol, min_depth, max_depth);
+  hipStream_t st = (hipStream_t)s;
+  const int64_t L = t->L;
+  const int q = t->q, dof = t->dof;
+  const size_t row = (size_t)dof * t->nc;
+  // 1. the criterion per leaf
+  std::vector<double> tail(L);
+  {
+    DevBuf d;
+    if (int rc = dev_alloc(d, sizeof(double) * L, st)) return rc;
+    HIP_TRY(launch_adapt_tail(t->d_coeff, L, dof, q, d.as<double>(), st));
+    HIP_TRY(hipMemcpyAsync(tail.data(), d.p, sizeof(double) * L, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  std::vector<uint8_t> refine(L);
+  for (int64_t j = 0; j < L; ++j)
+    refine[j] = tail[j] > tol && (int)t->h_info[j].w < max_depth ? 1 : 0;
+  // 2. complete sibling groups that may merge: 8 consecutive leaves of one
+  //    parent, none to be refined, the parent not above min_depth
+  std::vector<int64_t> cand;
+  for (int64_t j = 0; j + 8 <= L; ++j) {
+    const uint4 f = t->h_info[j];
+    if ((int)f.w <= min_depth || (f.x & 1) || (f.y & 1) || (f.z & 1)) continue;
+    bool ok = true;
+    for (uint32_t c = 0; c < 8 && ok; ++c) {
+      const uint4 g = t->h_info[j + c];
+      ok = g.w == f.w && g.x == f.x + (c & 1) && g.y == f.y + ((c >> 1) & 1) &&
+           g.z == f.z + ((c >> 2) & 1) && !refine[j + c];
+    }
+    if (ok) {
+      cand.push_back(j);
+      j += 7;
+    }
+  }
+  // 3. the parents' fits from their children (restriction) and their tails
+  DevBuf pcoef;
+  std::vector<uint8_t> merge(cand.size(), 0);
+  if (!cand.empty()) {
+    const int64_t nc = (int64_t)cand.size();
+    std::vector<int32_t> src(nc);
+    std::vector<uint4> info(nc);
+    for (int64_t k = 0; k < nc; ++k) {
+      const uint4 f = t->h_info[cand[k]];
+      src[k] = (int32_t)cand[k];
+      info[k] = make_uint4(f.x / 2, f.y / 2, f.z / 2, f.w - 1);
+    }
+    if (int rc = dev_alloc(pcoef, sizeof(double) * row * nc, st)) return rc;
+    if (int rc = resample(t, src, info, 1, pcoef.as<double>(), st)) return rc;
+    std::vector<double> ptail(nc);
+    DevBuf d;
+    if (int rc = dev_alloc(d, sizeof(double) * nc, st)) return rc;
+    HIP_TRY(launch_adapt_tail(pcoef.as<double>(), nc, dof, q, d.as<double>(), st));
+    HIP_TRY(hipMemcpyAsync(ptail.data(), d.p, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int64_t k = 0; k < nc; ++k) merge[k] = ptail[k] <= tol ? 1 : 0;
+  }
+  // 4. refine and merge
+  std::vector<NewLeaf> leaves;
+  leaves.reserve(L);
+  bool changed = false;
+  {
+    size_t k = 0;
+    for (int64_t j = 0; j < L; ++j) {
+      if (k < cand.size() && cand[k] == j) {
+        if (merge[k]) {
+          const uint4 f = t->h_info[j];
+          leaves.push_back(NewLeaf{{f.x / 2, f.y / 2, f.z / 2}, (uint8_t)(f.w - 1), 2, (int64_t)k});
+          j += 7;
+          ++k;
+          changed = true;
+          continue;
+        }
+        ++k;
+      }
+      if (refine[j]) {
+        push_children(t, j, leaves);
+        changed = true;
+      } else {
+        leaves.push_back(keep_leaf(t, j));
+      }
+    }
+  }
+  if (changed)
+    if (int rc = rebuild(t, leaves, pcoef.as<double>(), st)) return rc;
+  // 5. 2:1 balance: split every leaf more than one level coarser than a
+  //    neighbour, until none is
+  for (;;) {
+    const int64_t Lb = t->L;
+    std::vector<uint8_t> split(Lb);
+    {
+      DevBuf d;
+      if (int rc = dev_alloc(d, Lb, st)) return rc;
+      HIP_TRY(hipMemsetAsync(d.p, 0, Lb, st));
+      HIP_TRY(launch_balance_probe(t->view(), bc, d.as<uint8_t>(), st));
+      HIP_TRY(hipMemcpyAsync(split.data(), d.p, Lb, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+    std::vector<NewLeaf> next;
+    bool any = false;
+    for (int64_t j = 0; j < Lb; ++j) {
+      if (split[j]) {
+        push_children(t, j, next);
+        any = true;
+      } else {
+        next.push_back(keep_leaf(t, j));
+      }
+    }
+    if (!any) break;
+    if (int rc = rebuild(t, next, nullptr, st)) return rc;
+  }
+  return TBX_OK;
+}
+
+int tbx_tree_get_leaves(const tbx_tree *t, uint32_t *anchor, uint8_t *depth) {
+  if (!t || !anchor || !depth) return fail(TBX_EINVAL, "tbx_tree_get_leaves: null argument");
+  for (int64_t j = 0; j < t->L; ++j) {
+    const uint4 f = t->h_info[j];
+    anchor[3 * j] = f.x;
+    anchor[3 * j + 1] = f.y;
+    anchor[3 * j + 2] = f.z;
+    depth[j] = (uint8_t)f.w;
+  }
   return TBX_OK;
 }
 

@@ -241,6 +241,22 @@ hipError_t launch_diff_norm2(const double *x, const double *y, int64_t n, double
 constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
 hipError_t noalloc_scratch(void **ptr, size_t *bytes);
 
+// On-device adaptivity (tbx_adapt.hip).
+hipError_t launch_adapt_tail(const double *coeff, int64_t L, int dof, int q, double *tail,
+                             hipStream_t s);
+// vals [nnew][dof][(q+1)^3]: each new leaf's stretched grid nodes (node[q+1])
+// evaluated in the source leaf src_leaf[n] (restrict_children: in the child
+// src_leaf[n] + octant of the node).
+hipError_t launch_leaf_resample(const TreeView &src, const int32_t *src_leaf,
+                                const uint4 *info_new, int restrict_children, int64_t nnew,
+                                const double *node, double *vals, hipStream_t s);
+// split[l] = 1 for every leaf l more than one level coarser than a leaf
+// whose same-size neighbour cell it holds (split must start at 0).
+hipError_t launch_balance_probe(const TreeView &t, int bc, uint8_t *split, hipStream_t s);
+// dst row r = src row map[r] where map[r] >= 0.
+hipError_t launch_gather_rows(const double *src, const int64_t *map, int64_t nrows, int rowlen,
+                              double *dst, hipStream_t s);
+
 // Degree-independent launchers (tbx_common.hip).
 hipError_t launch_locate(const TreeView &t, const double *pts, int64_t n,
                          int bc, int32_t *leaf, hipStream_t s);

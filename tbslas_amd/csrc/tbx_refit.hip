@@ -166,11 +166,10 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
   __shared__ double sa[P];
   __shared__ double sb[P];
   __shared__ double sT[D * D];  // sT[k * D + a] = S[k][a]
-  const int64_t r = blockIdx.x;
-  if (r >= rows) return;
   const int tid = threadIdx.x;
   for (int t = tid; t < D * D; t += kRefitThreads) sT[t] = tab.T[t / D][t % D];
   __syncthreads();
+  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
   const double *v = V + (r / dof) * (int64_t)P * dof + (r % dof) * cs;
   for (int t = tid; t < D * D; t += kRefitThreads) {  // t = c * D + b
     double x[D];
@@ -213,6 +212,8 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
     }
   }
   for (int m = NC + tid; m < ldw; m += kRefitThreads) w[m] = 0.0;
+  __syncthreads();  // the next row reuses sa and sb
+  }
 }
 
 // The transpose: v = B u per row (u rows of stride KP in packed order).
@@ -224,11 +225,10 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__
   __shared__ double sa[P];
   __shared__ double sb[P];
   __shared__ double sT[D * D];
-  const int64_t r = blockIdx.x;
-  if (r >= rows) return;
   const int tid = threadIdx.x;
   for (int t = tid; t < D * D; t += kRefitThreads) sT[t] = tab.T[t / D][t % D];
   __syncthreads();
+  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
   const double *u = U + r * KP;
   for (int t = tid; t < D * D; t += kRefitThreads) {  // t = j * D + k
     const int j = t / D, k = t % D;
@@ -270,6 +270,8 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__
       for (int k = 0; k < D; ++k) s = __builtin_fma(sT[k * D + a], x[k], s);
       v[t * D + a] = s;
     }
+  }
+  __syncthreads();  // the next row reuses sa and sb
   }
 }
 
@@ -379,27 +381,39 @@ hipError_t launch_refit_gemm(const double *A, int64_t lda, int64_t rows, int K, 
                              const double *H, int np, double *C, int64_t ldc, int ncols,
                              float *C32, hipStream_t s) {
   if (rows <= 0) return hipSuccess;
-  const int64_t ntiles = ((rows + kGemmBM - 1) / kGemmBM) * (np / kGemmBN);
-  const int64_t per_xcd = (ntiles + 7) / 8;
-  if (8 * per_xcd > INT32_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_refit_gemm, dim3((unsigned)(8 * per_xcd)), dim3(kGemmThreads), 0, s, A,
-                     lda, rows, K, KP, H, np, C, ldc, ncols, C32, ntiles, per_xcd);
-  return hipGetLastError();
+  // at most 2^22 tiles (2^30 work-items) per dispatch
+  const int64_t chunk = ((int64_t)1 << 22) / (np / kGemmBN) * kGemmBM;
+  for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+    const int64_t nr = rows - r0 < chunk ? rows - r0 : chunk;
+    const int64_t ntiles = ((nr + kGemmBM - 1) / kGemmBM) * (np / kGemmBN);
+    const int64_t per_xcd = (ntiles + 7) / 8;
+    hipLaunchKernelGGL(k_refit_gemm, dim3((unsigned)(8 * per_xcd)), dim3(kGemmThreads), 0, s,
+                       A + r0 * lda, lda, nr, K, KP, H, np, C + r0 * ldc, ldc, ncols,
+                       C32 ? C32 + r0 * ldc : nullptr, ntiles, per_xcd);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 #define TBX_REFIT_CASES(X) \
   X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17)
 
+// A dispatch holds at most 2^32 - 1 work-items: the row kernels run at most
+// kRowBlocks workgroups, each taking every kRowBlocks-th row.
+constexpr int64_t kRowBlocks = 1 << 20;
+
 hipError_t launch_refit_project(int d, const double *V, int64_t rows, int dof, int64_t cs,
                                 int64_t ps, const RefitTable &tab, double *W, int ldw,
                                 hipStream_t s) {
   if (rows <= 0) return hipSuccess;
-  if (rows > INT32_MAX || dof < 1) return hipErrorInvalidValue;
+  if (dof < 1) return hipErrorInvalidValue;
+  const unsigned nb = (unsigned)(rows < kRowBlocks ? rows : kRowBlocks);
   switch (d) {
 #define X(D)                                                                              \
   case D:                                                                                 \
-    hipLaunchKernelGGL(k_refit_project<D>, dim3((unsigned)rows), dim3(kRefitThreads), 0, s, \
-                       V, rows, dof, cs, ps, tab, W, ldw);                                \
+    hipLaunchKernelGGL(k_refit_project<D>, dim3(nb), dim3(kRefitThreads), 0, s, V, rows, dof, \
+                       cs, ps, tab, W, ldw);                                              \
     break;
     TBX_REFIT_CASES(X)
 #undef X
@@ -412,12 +426,12 @@ hipError_t launch_refit_project(int d, const double *V, int64_t rows, int dof, i
 hipError_t launch_refit_expand(int d, const double *U, int64_t rows, const RefitTable &tab,
                                int KP, double *V, hipStream_t s) {
   if (rows <= 0) return hipSuccess;
-  if (rows > INT32_MAX) return hipErrorInvalidValue;
+  const unsigned nb = (unsigned)(rows < kRowBlocks ? rows : kRowBlocks);
   switch (d) {
 #define X(D)                                                                             \
   case D:                                                                                \
-    hipLaunchKernelGGL(k_refit_expand<D>, dim3((unsigned)rows), dim3(kRefitThreads), 0, s, \
-                       U, rows, tab, KP, V);                                             \
+    hipLaunchKernelGGL(k_refit_expand<D>, dim3(nb), dim3(kRefitThreads), 0, s, U, rows, tab, \
+                       KP, V);                                                           \
     break;
     TBX_REFIT_CASES(X)
 #undef X

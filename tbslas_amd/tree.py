@@ -263,6 +263,22 @@ class Tree:
                                           _stream()))
         return tuple(float(v) for v in res)
 
+    def adapt(self, tol: float, min_depth: int, max_depth: int, bc: int = BC_FREESPACE) -> int:
+        """RefineTree + Balance21 on the device (tbx_tree_adapt): split
+        leaves whose top-shell tail exceeds tol (below max_depth), merge
+        sibling groups whose parent fit's tail is <= tol (above min_depth),
+        then 2:1-balance.  Updates anchor, depth and L; returns L."""
+        check(lib().tbx_tree_adapt(self._h, float(tol), int(min_depth), int(max_depth), bc,
+                                   _stream()))
+        L = C.c_int64()
+        q, dof, md = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().tbx_tree_info(self._h, C.byref(L), C.byref(q), C.byref(dof), C.byref(md)))
+        self.L = int(L.value)
+        self.anchor = np.empty((self.L, 3), dtype=np.uint32)
+        self.depth = np.empty(self.L, dtype=np.uint8)
+        check(lib().tbx_tree_get_leaves(self._h, _ptr(self.anchor), _ptr(self.depth)))
+        return self.L
+
     def write_vtu(self, prefix: str, order: int = 4) -> None:
         """Write2File: <prefix>.pvtu and <prefix>_000000.vtu with the tree's
         values on order^3 uniform points per leaf (VTK hexahedra)."""

@@ -163,3 +163,20 @@ def test_harness_errors_equal_the_oracle(torch, tb, orc, cv):
     ev = lambda p: orc.eval_tree(oc, p)[0]
     e_out = metrics.check_cheb_output(anchor, depth, q, ev, lambda p: c(p, tn * dt))
     assert np.allclose(e_out, r["out"], rtol=1e-9, atol=0), (e_out, r["out"])
+
+
+def test_zalesak_adaptive_tree_follows_the_sphere(torch, tb, cv):
+    # the reference's adaptive Zalesak run (conv_zal.py:47-98: tol 1e-10,
+    # dt 0.0628, the tree refined and balanced after every step) at maximum
+    # depth 5 for one quarter revolution: the tree changes from step to step
+    # and stays balanced, and q = 8 beats q = 3 (the reference's table runs
+    # depths 5 and 7 over 5 revolutions: `python -m tbslas_amd.convergence
+    # zalesak_adaptive`)
+    rows = {}
+    for q in (3, 8):
+        r = cv.run_advection_adaptive(2, q, 6.28e-2, 25, 1e-10, 5)
+        print(r)
+        assert r["leaves_max"] > r["leaves_min"]
+        assert r["max_depth"] == 5
+        rows[q] = r
+    assert rows[8]["out"][1] < rows[3]["out"][1]

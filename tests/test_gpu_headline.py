@@ -21,6 +21,8 @@ import os
 import numpy as np
 import pytest
 
+from conftest import wrap_periodic
+
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(1200)]
 
 STEPS = 3
@@ -47,7 +49,10 @@ def case(tb, orc):
     ov, oc = orc.Tree(anchor, depth, q, 3, vcf), orc.Tree(anchor, depth, q, 1, ccf)
     print(f"# headline workload: {len(depth)} leaves, {len(depth) * (q + 1) ** 3} grid points",
           flush=True)
-    dep = orc.traj_rk2(ov, orc.collect_grid_points(oc), dt, 0.0, 1, bc, g=g)
+    # ComputeTrajRK2's end points; SolveSemilagRK2's concentration functor
+    # then wraps them in place (EvalTree, tree_functor.h:442-449), and the
+    # step's departure points are the wrapped ones
+    dep = wrap_periodic(orc.traj_rk2(ov, orc.collect_grid_points(oc), dt, 0.0, 1, bc, g=g))
     leaf = orc.locate(oc, dep, bc)
     M = orc.pt2coeff(q)
     states = []
