@@ -335,12 +335,21 @@ int tbx_batch_get_config(const tbx_batch *b, int *cfg);
 /* Times each configuration (ms[6], may be NULL), keeps the fastest. */
 int tbx_batch_tune(tbx_batch *b, double *ms, tbx_stream s);
 int tbx_batch_step(tbx_batch *b, int nsteps, tbx_stream s);
+/* Carry an in-situ step of (vel, conc) -- for example many tracers on one
+ * trajectory set -- in the batch's step graphs, ahead of the persistent
+ * kernel: each tbx_batch_step replay then advances conc and every problem
+ * of the batch in one graph launch.  The velocity is stationary (no time
+ * factor), flags give the arithmetic, the batch's bc applies.  The step's
+ * buffers are made here (the graphs allocate nothing); conc and vel must
+ * outlive the batch. */
+int tbx_batch_attach_step(tbx_batch *b, const tbx_tree *vel, tbx_tree *conc, double dt,
+                          int num_rk_step, uint32_t flags);
 /* Current coefficients of problem p, host [L][1][Nc] (FP32 state). */
 int tbx_batch_get_coeffs(const tbx_batch *b, int p, double *coeff, tbx_stream s);
 
 /* ---- adjoint (no counterpart in the reference) --------------------------
- * The step c' = R E_D(V) c of tbx_semilag_insitu (one RK2 substep, a single
- * velocity tree, optionally scaled by g) is linear in c; its cotangents are
+ * The step c' = R E_D(V) c of tbx_semilag_insitu (a single velocity tree,
+ * optionally scaled by g) is linear in c; its cotangents are
  *   dc = E_D^T R^T lam'                the transposed interpolation, a
  *                                      per-leaf scatter-add into coefficients
  *   dV = sum_p a_p . dD_p/dV           basis derivatives at the departure
@@ -353,7 +362,8 @@ int tbx_batch_get_coeffs(const tbx_batch *b, int p, double *coeff, tbx_stream s)
  * (device arrays [L][dof][Nc]); grad_vel (device [L_vel][3][Nc_vel], may be
  * NULL) is ADDED to.  dep (device [L*P][3], may be NULL): the departure
  * points that forward step wrote (its dep output); read back instead of
- * integrating the trajectories again.  num_rk_step must be 1. */
+ * integrating the trajectories again.  num_rk_step: the forward step's RK
+ * substeps, 1..16 (the velocity cotangent runs back through them). */
 int tbx_semilag_insitu_vjp(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
                            const tbx_tree *conc, int timestep, double dt,
                            int num_rk_step, int bc, uint32_t flags,
@@ -428,8 +438,12 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
  * components (many tracers on one trajectory set) has its components
  * contracted on the matrix cores in TBX_FP64 and TBX_FP32 steps (regrouped
  * workgroups of 512 threads in FP64 and 256 in FP32, whatever the step
- * shape); 0 keeps the vector contraction.  The two agree to rounding (summation order). */
-enum { TBX_OPT_STEP_SHAPE = 1, TBX_OPT_TRACER_MFMA = 2 };
+ * shape); 0 keeps the vector contraction.  The two agree to rounding (summation order).
+ * TBX_OPT_GRID_LDS (default 1): the grid-leaf step on a uniform tree of 8^3-sample
+ * leaves whose velocity and concentration trees share their leaves stages each
+ * sibling block of 8 leaves in LDS, in FP32 steps (1) or in every mode (2); 0
+ * gathers every stencil from global memory.  All give the same bits. */
+enum { TBX_OPT_STEP_SHAPE = 1, TBX_OPT_TRACER_MFMA = 2, TBX_OPT_GRID_LDS = 3 };
 int tbx_set_option(int option, int value);
 int tbx_get_option(int option, int *value);
 

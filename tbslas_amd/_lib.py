@@ -23,6 +23,7 @@ MAX_Q = 16
 MAX_DEPTH = 21
 OPT_STEP_SHAPE = 1
 OPT_TRACER_MFMA = 2
+OPT_GRID_LDS = 3
 VEL_TREE = 0
 VEL_EXTRAP = 1
 VEL_SET = 2
@@ -125,6 +126,7 @@ _SIGS = {
     "tbx_batch_get_config": (_I32, [_P, C.POINTER(_I32)]),
     "tbx_batch_tune": (_I32, [_P, _P, _P]),
     "tbx_batch_step": (_I32, [_P, _I32, _P]),
+    "tbx_batch_attach_step": (_I32, [_P, _P, _P, _D, _I32, _U32]),
     "tbx_batch_get_coeffs": (_I32, [_P, _I32, _P, _P]),
     "tbx_new_nodes": (_I32, [_I32, _P]),
     "tbx_pt2coeff_matrix": (_I32, [_I32, _P]),

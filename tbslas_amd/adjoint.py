@@ -1,8 +1,8 @@
 """Adjoint of the semi-Lagrangian step over the libtbx C ABI.
 
 The reference has no adjoint.  The step c' = R E_D(V) c (tbx_semilag_insitu:
-departure points D of the grid points by one RK2 substep through the
-velocity tree V, interpolation E_D, per-leaf refit R) is differentiated by
+departure points D of the grid points by RK2 substeps through the velocity
+tree V, interpolation E_D, per-leaf refit R) is differentiated by
 hand in the library (tbx_semilag_insitu_vjp, tbx_adjoint_run).  The
 torch.autograd.Function here wraps the same C entry points; it exists to
 check the hand-written adjoint against finite differences of the forward.
@@ -17,12 +17,13 @@ from .tree import Tree, _mode, _ptr, _require_dev, _stream
 
 def semilag_insitu_vjp(vel: Tree, conc: Tree, timestep: int, dt: float, lam_next,
                        bc: int = BC_FREESPACE, mode: str = "fp64", g=None, grad_vel=None,
-                       dep=None):
+                       dep=None, num_rk_step: int = 1):
     """Cotangent of one in-situ step taken from conc's current coefficients:
     returns dL/dc for lam_next = dL/dc' (cuda [L, dof, Nc]); grad_vel (cuda
     [L_vel, 3, Nc]) is added to when given.  dep: the departure points the
     forward step wrote (semilag_insitu(..., dep=)), read back instead of
-    integrating the trajectories again."""
+    integrating the trajectories again.  num_rk_step: the forward step's RK
+    substeps."""
     import torch
 
     _require_dev(lam_next, torch.float64, "lam_next")
@@ -38,7 +39,8 @@ def semilag_insitu_vjp(vel: Tree, conc: Tree, timestep: int, dt: float, lam_next
             raise ValueError("dep must hold the L*P departure points of the step")
     lam_out = torch.empty_like(lam_next)
     cb = tfac(g)
-    check(lib().tbx_semilag_insitu_vjp(vel.handle, cb, None, conc.handle, timestep, dt, 1, bc,
+    check(lib().tbx_semilag_insitu_vjp(vel.handle, cb, None, conc.handle, timestep, dt,
+                                       num_rk_step, bc,
                                        _mode(mode), _ptr(dep), _ptr(lam_next), _ptr(lam_out),
                                        _ptr(grad_vel), _stream()))
     return lam_out

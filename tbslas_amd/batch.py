@@ -20,6 +20,7 @@ class Batch:
         if not (len(vel) == len(conc) == len(dt)) or not vel:
             raise ValueError("one velocity tree, concentration tree and dt per problem")
         self.vel, self.conc = list(vel), list(conc)  # keep the trees alive
+        self._attached = []
         n = len(vel)
         vh = (C.c_void_p * n)(*[t.handle.value for t in vel])
         ch = (C.c_void_p * n)(*[t.handle.value for t in conc])
@@ -54,6 +55,16 @@ class Batch:
         ms = np.zeros(len(CONFIGS))
         check(lib().tbx_batch_tune(self._h, ms.ctypes.data, _stream()))
         return ms
+
+    def attach_step(self, vel: Tree, conc: Tree, dt: float, num_rk_step: int = 1,
+                    mode: str = "fp32") -> None:
+        """Carry an in-situ step of (vel, conc) (stationary velocity) in every
+        replay of the batch's step graph, ahead of the persistent kernel."""
+        from .tree import _mode
+
+        self._attached.append((vel, conc))  # keep the trees alive
+        check(lib().tbx_batch_attach_step(self._h, vel.handle, conc.handle, dt, num_rk_step,
+                                          _mode(mode)))
 
     def step(self, nsteps: int = 1) -> None:
         check(lib().tbx_batch_step(self._h, nsteps, _stream()))

@@ -266,8 +266,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_conc(VelSource vel, TreeView conc
     } else {
       double x, y, z;
       arrival_point(conc, a, i, x, y, z);
-      rk2_substep<D, MODE, kVelTree>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[0], a.g[1],
-                                     a.ts[0], a.ts[1], x, y, z);
+#pragma unroll 1
+      for (int s = 0; s < a.nsub; ++s)
+        rk2_substep<D, MODE, kVelTree>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[2 * s],
+                                       a.g[2 * s + 1], a.ts[2 * s], a.ts[2 * s + 1], x, y, z);
       if (a.bc == TBX_BC_PERIODIC) {
         x = wrap1(x);
         y = wrap1(y);
@@ -313,64 +315,120 @@ __global__ __launch_bounds__(BLOCK) void k_adj_conc(VelSource vel, TreeView conc
 // to degree 8 (the register budget halves to 256).
 constexpr int kVelWaves = D <= 9 ? 2 : 1;
 
-// Velocity cotangent: for D = x + tau g2 V(h), h = x + (tau/2) g1 V(x),
-//   a . dD = tau g2 a . dV(h) + tau g2 (tau/2) g1 (J_V(h)^T a) . dV(x),
-// scattered into G[leaf][k][m] at h (in h's leaf) and at x (in x's leaf).
+// Velocity cotangent through the num_rk_step substeps.  Substep s maps x_s
+// to x_{s+1} = x_s + tau g2_s V(h_s), h_s = x_s + (tau/2) g1_s V(x_s)
+// (traj.inc:20-46; the periodic wraps are translations), so with a_{s+1}
+// the cotangent of x_{s+1} (a_S = A, the departure point's):
+//   a . dx_{s+1} gives   tau g2_s a_{s+1} . dV(h_s)
+//                      + w_s . dV(x_s),  w_s = tau g2_s (tau/2) g1_s J_V(h_s)^T a_{s+1}
+//   a_s = a_{s+1} + tau g2_s J_V(h_s)^T a_{s+1} + J_V(x_s)^T w_s,
+// each dV scattered into G[leaf][k][m] at its point (in its leaf).  The
+// substeps run backwards; x_s is integrated again from the arrival point
+// (s substeps) for each one.  Before the work at h_s and again before the
+// work at x_s the workgroup's points are regrouped by the leaf they fall in
+// (as in the concentration kernel), carrying their state along, so a
+// wavefront's scatter reads and adds one leaf's coefficients at a time.
 template <int MODE>
-__global__ __launch_bounds__(kBlock, kVelWaves) void k_adj_vel(VelSource vel, TreeView conc, StepArgs a,
-                                                    const double *__restrict__ A,
-                                                    double *__restrict__ G) {
-  const int64_t i0 = (int64_t)blockIdx.x * kBlock;
-  const int64_t gi = i0 + threadIdx.x;
-  const bool live = gi < a.npts;
-  const int64_t i = live ? gi : a.npts - 1;  // idle lanes follow the last point
+__global__ __launch_bounds__(kAdjBlock, kVelWaves) void k_adj_vel(VelSource vel, TreeView conc,
+                                                                  StepArgs a,
+                                                                  const double *__restrict__ A,
+                                                                  double *__restrict__ G) {
+  __shared__ RegroupLds<kAdjBlock, 12> sh;
+  const uint32_t b = blockIdx.x;
+  const int64_t lb = (int64_t)(b & 7u) * a.per_xcd + (int64_t)(b >> 3);
+  if (lb >= a.nblocks) return;  // uniform over the workgroup
+  const int64_t i0 = lb * kAdjBlock;
   const TreeView &V = vel.t[0];
-  double x, y, z;
-  arrival_point(conc, a, i, x, y, z);
-  if (a.bc == TBX_BC_PERIODIC) {
-    x = wrap1(x);
-    y = wrap1(y);
-    z = wrap1(z);
+  int orig = (int)threadIdx.x;
+  double ax, ay, az;
+  {
+    const int64_t gi = i0 + orig;
+    const bool live = gi < a.npts;
+    ax = live ? A[3 * gi + 0] : 0.0;  // idle lanes carry zero weights
+    ay = live ? A[3 * gi + 1] : 0.0;
+    az = live ? A[3 * gi + 2] : 0.0;
   }
-  // the forward's midpoint (rk2_substep, stage 1)
-  double v0, v1, v2;
-  double xs = x, ys = y, zs = z;
-  eval_velocity<D, MODE, kVelTree>(vel, 1, a.ts[0], a.scaled, a.g[0], a.bc, xs, ys, zs, v0, v1,
-                                   v2);
-  double hx = x + a.htau * v0, hy = y + a.htau * v1, hz = z + a.htau * v2;
-  if (a.bc == TBX_BC_PERIODIC) {
-    hx = wrap1(hx);
-    hy = wrap1(hy);
-    hz = wrap1(hz);
-  }
-  const double ax = live ? A[3 * i + 0] : 0.0;
-  const double ay = live ? A[3 * i + 1] : 0.0;
-  const double az = live ? A[3 * i + 2] : 0.0;
-  const double g1 = a.scaled ? a.g[0] : 1.0, g2 = a.scaled ? a.g[1] : 1.0;
-  AdjBasis bh;
-  bh.build(V, hx, hy, hz);
-  // J^T a at h: (J^T a)_k = sum_c a_c dV_c/dy_k
-  double jx = 0.0, jy = 0.0, jz = 0.0;
-  for_each_wave_leaf(bh.leaf, [&](int leaf) {
 #pragma unroll 1
-    for (int c = 0; c < 3; ++c) {
-      double gx, gy, gz;
-      grad3(V.coeff + ((size_t)leaf * 3 + c) * NC, bh, gx, gy, gz);
-      const double ac = c == 0 ? ax : c == 1 ? ay : az;
-      jx = __builtin_fma(ac, gx, jx);
-      jy = __builtin_fma(ac, gy, jy);
-      jz = __builtin_fma(ac, gz, jz);
+  for (int s = a.nsub - 1; s >= 0; --s) {
+    const int64_t i = i0 + orig < a.npts ? i0 + orig : a.npts - 1;
+    double x, y, z;
+    arrival_point(conc, a, i, x, y, z);
+#pragma unroll 1
+    for (int r = 0; r < s; ++r)
+      rk2_substep<D, MODE, kVelTree>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[2 * r],
+                                     a.g[2 * r + 1], a.ts[2 * r], a.ts[2 * r + 1], x, y, z);
+    if (a.bc == TBX_BC_PERIODIC) {
+      x = wrap1(x);
+      y = wrap1(y);
+      z = wrap1(z);
     }
-  });
-  const double fh = a.tau * g2, fx = a.tau * g2 * a.htau * g1;
-  scatter_transposed<32>(G, 3, bh, [&](int c) {
-    return fh * (c == 0 ? ax : c == 1 ? ay : az);
-  });
-  AdjBasis bx;
-  bx.build(V, x, y, z);
-  scatter_transposed<32>(G, 3, bx, [&](int c) {
-    return fx * (c == 0 ? jx : c == 1 ? jy : jz);
-  });
+    // the forward's midpoint of substep s (rk2_substep, stage 1)
+    double v0, v1, v2;
+    double xs = x, ys = y, zs = z;
+    eval_velocity<D, MODE, kVelTree>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, xs, ys, zs,
+                                     v0, v1, v2);
+    double hx = x + a.htau * v0, hy = y + a.htau * v1, hz = z + a.htau * v2;
+    if (a.bc == TBX_BC_PERIODIC) {
+      hx = wrap1(hx);
+      hy = wrap1(hy);
+      hz = wrap1(hz);
+    }
+    const double g1 = a.scaled ? a.g[2 * s] : 1.0, g2 = a.scaled ? a.g[2 * s + 1] : 1.0;
+    const double fh = a.tau * g2, fx = a.tau * g2 * a.htau * g1;
+    // work at h: regroup by h's leaf
+    int lh = locate(V, hx, hy, hz);
+    double ph[9] = {hx, hy, hz, ax, ay, az, x, y, z};
+    regroup(sh, orig, lh, ph);
+    ax = ph[3];
+    ay = ph[4];
+    az = ph[5];
+    double jx = 0.0, jy = 0.0, jz = 0.0;
+    {
+      AdjBasis bh;
+      bh.build(V, ph[0], ph[1], ph[2], lh);
+      // J^T a at h: (J^T a)_k = sum_c a_c dV_c/dy_k
+      for_each_wave_leaf(bh.leaf, [&](int leaf) {
+#pragma unroll 1
+        for (int c = 0; c < 3; ++c) {
+          double gx, gy, gz;
+          grad3(V.coeff + ((size_t)leaf * 3 + c) * NC, bh, gx, gy, gz);
+          const double ac = c == 0 ? ax : c == 1 ? ay : az;
+          jx = __builtin_fma(ac, gx, jx);
+          jy = __builtin_fma(ac, gy, jy);
+          jz = __builtin_fma(ac, gz, jz);
+        }
+      });
+      scatter_transposed<32>(G, 3, bh, [&](int c) {
+        return fh * (c == 0 ? ax : c == 1 ? ay : az);
+      });
+    }
+    // work at x: regroup by x's leaf
+    int lx = locate(V, ph[6], ph[7], ph[8]);
+    double px[12] = {ph[6], ph[7], ph[8], fx * jx, fx * jy, fx * jz, ax, ay, az, jx, jy, jz};
+    regroup(sh, orig, lx, px);
+    const double wx = px[3], wy = px[4], wz = px[5];
+    AdjBasis bx;
+    bx.build(V, px[0], px[1], px[2], lx);
+    scatter_transposed<32>(G, 3, bx, [&](int c) { return c == 0 ? wx : c == 1 ? wy : wz; });
+    if (s > 0) {
+      // a_s = a_{s+1} + tau g2 J(h)^T a_{s+1} + J(x)^T w
+      double kx = 0.0, ky = 0.0, kz = 0.0;
+      for_each_wave_leaf(bx.leaf, [&](int leaf) {
+#pragma unroll 1
+        for (int c = 0; c < 3; ++c) {
+          double gx, gy, gz;
+          grad3(V.coeff + ((size_t)leaf * 3 + c) * NC, bx, gx, gy, gz);
+          const double wc = c == 0 ? wx : c == 1 ? wy : wz;
+          kx = __builtin_fma(wc, gx, kx);
+          ky = __builtin_fma(wc, gy, ky);
+          kz = __builtin_fma(wc, gz, kz);
+        }
+      });
+      ax = px[6] + fh * px[9] + kx;
+      ay = px[7] + fh * px[10] + ky;
+      az = px[8] + fh * px[11] + kz;
+    }
+  }
 }
 
 }  // namespace
@@ -402,7 +460,7 @@ hipError_t TBX_CAT(launch_adj_vel_q, TBX_Q)(int mode, const VelSource &vel,
                                             const TreeView &conc, const StepArgs &a,
                                             const double *A, double *G, hipStream_t s) {
   if (a.npts <= 0) return hipSuccess;
-  const dim3 g((unsigned)((a.npts + kBlock - 1) / kBlock)), blk(kBlock);
+  const dim3 g((unsigned)(8 * a.per_xcd)), blk(kAdjBlock);
   if (mode == kFma64)
     hipLaunchKernelGGL((k_adj_vel<kFma64>), g, blk, 0, s, vel, conc, a, A, G);
   else if (mode == kExact64)

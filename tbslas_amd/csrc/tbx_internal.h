@@ -169,6 +169,8 @@ struct GridStepArgs {
   int32_t nsub;
   int32_t bc;
   int32_t scaled;
+  int32_t blk;   // uniform tree of 8^3-sample leaves shared by velocity and
+                 // concentration: sibling blocks staged in LDS
   double tau, htau;
   double g[2 * kMaxSub];
   double node[16];  // a / (N - 1)

@@ -49,10 +49,47 @@ def _setup(tb, seed, q, dof):
     return anchor, depth, vel, conc, c0
 
 
-def _forward(tb, vel, conc, c, timestep, dt, bc, g=None):
+def _forward(tb, vel, conc, c, timestep, dt, bc, g=None, nsub=1):
     conc.set_coeffs(c)
-    tb.semilag_insitu(vel, conc, timestep, dt, 1, bc, "fp64", g=g)
+    tb.semilag_insitu(vel, conc, timestep, dt, nsub, bc, "fp64", g=g)
     return conc.get_coeffs_device()
+
+
+@pytest.mark.parametrize("nsub,bc,timed", [(2, 0, False), (3, 1, True)])
+def test_vjp_several_substeps(torch, tb, nsub, bc, timed):
+    # a forward step of nsub RK2 substeps: the dot-product identity for the
+    # concentration and central differences for the velocity cotangent,
+    # which runs back through every substep
+    from tbslas_amd.adjoint import semilag_insitu_vjp
+
+    g = (lambda t: math.cos(math.pi * t)) if timed else None
+    _, _, vel, conc, c0 = _setup(tb, 63, 6, 1)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    c = torch.randn(c0.shape, dtype=torch.float64, device="cuda", generator=gen)
+    lam = torch.randn(c0.shape, dtype=torch.float64, device="cuda", generator=gen)
+    fc = _forward(tb, vel, conc, c, 2, 0.04, bc, g, nsub)
+    conc.set_coeffs(c)
+    ftl = semilag_insitu_vjp(vel, conc, 2, 0.04, lam, bc, "fp64", g=g, num_rk_step=nsub)
+    lhs, rhs = float((lam * fc).sum()), float((ftl * c).sum())
+    assert abs(lhs - rhs) <= 1e-13 * float(lam.norm() * fc.norm()), (lhs, rhs)
+    v0 = vel.get_coeffs_device().clone()
+    c = torch.from_numpy(c0).cuda()
+    gv = torch.zeros_like(v0)
+    conc.set_coeffs(c)
+    semilag_insitu_vjp(vel, conc, 2, 0.04, lam, bc, "fp64", g=g, grad_vel=gv, num_rk_step=nsub)
+
+    def loss(v):
+        vel.set_coeffs(v)
+        return float((lam * _forward(tb, vel, conc, c, 2, 0.04, bc, g, nsub)).sum())
+
+    eps = 1e-6
+    for k in range(4):
+        d = torch.randn(v0.shape, dtype=torch.float64, device="cuda", generator=gen)
+        d = d / d.abs().max()
+        fd = (loss(v0 + eps * d) - loss(v0 - eps * d)) / (2 * eps)
+        an = float((gv * d).sum())
+        assert abs(fd - an) <= 1e-6 * abs(an) + 1e-9 * float(gv.norm() * d.norm()), (k, fd, an)
+    vel.set_coeffs(v0)
 
 
 @pytest.mark.parametrize("bc,timed", [(0, False), (1, True)])

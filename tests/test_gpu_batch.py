@@ -229,3 +229,41 @@ def test_batch_preserves_a_constant_on_the_largest_problem(torch, tb):
         assert np.abs(cf[:, 0, 0] - 1.0).max() <= 1e-6
         assert np.abs(cf[:, 0, 1:]).max() <= 1e-6
     b.close()
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp64"])
+def test_attached_tracers_ride_the_batch_graph(torch, tb, mode):
+    # a 16-tracer concentration (matrix-core contraction) attached to a batch
+    # of small problems: every replay of the one step graph advances the
+    # tracers exactly as an eager in-situ step does and the problems exactly
+    # as the batch alone does; the replays alternate with eager work and
+    # synchronisations, and every replay is checked
+    from tbslas_amd import fields
+    from tbslas_amd.batch import Batch
+
+    probs = _small_problems(tb, n=6)
+    vel = [tb.Tree(a, d, q, 3, v) for a, d, q, v, c, dt in probs]
+    dts = [dt for *_, dt in probs]
+    q, T = 8, 16
+    anchor, depth = tb.uniform_leaves(3)
+    tv = tb.Tree(anchor, depth, q, 3, tb.fit_function(
+        lambda p: fields.rotation(p, 2 * math.pi), anchor, depth, q, 3))
+    centres, sigma = fields.tracer_blobs(T, 3)
+    ccf = tb.fit_function(lambda p: fields.gaussian_blobs(p, centres, sigma), anchor, depth, q, T)
+    tc = tb.Tree(anchor, depth, q, T, ccf)
+    ref = tb.Tree(anchor, depth, q, T, ccf)
+    alone = Batch(vel, [tb.Tree(a, d, q_, 1, c) for a, d, q_, v, c, dt in probs], dts, 0)
+    b = Batch(vel, [tb.Tree(a, d, q_, 1, c) for a, d, q_, v, c, dt in probs], dts, 0)
+    b.attach_step(tv, tc, 0.005, 1, mode)
+    probe = torch.rand((4096, 3), dtype=torch.float64, device="cuda")
+    for r in range(3):
+        b.step(1)
+        alone.step(1)
+        tb.semilag_insitu(tv, ref, 1, 0.005, 1, 0, mode)
+        ref.eval(probe.clone(), 0, "fp64")  # eager work between replays
+        torch.cuda.synchronize()
+        assert np.array_equal(tc.get_coeffs(), ref.get_coeffs()), r
+        for p in range(len(probs)):
+            assert np.array_equal(b.coeffs(p), alone.coeffs(p)), (r, p)
+    b.close()
+    alone.close()

@@ -103,3 +103,43 @@ def test_tricubic_full_size_step_properties(torch, tb):
     got = conc.get_samples()[:, 0, :]
     assert inside.mean() > 0.99
     assert np.abs(got[inside] - want[inside]).max() <= 1e-12
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_grid_step_lds_blocks_against_oracle(torch, tb, orc, bc):
+    # a uniform tree of 8^3-sample leaves shared by velocity and concentration
+    # runs the sibling-block kernel (samples staged in LDS; TBX_OPT_GRID_LDS 2
+    # takes it in every mode): TBX_EXACT bit-identical with the oracle, and
+    # every mode bit-identical with the global-memory kernel (option 0)
+    from tbslas_amd import _lib, fields
+
+    n = 8
+    anchor, depth = tb.uniform_leaves(3)
+    vs = tb.sample_function(lambda p: fields.taylor_green_2d3(p) + 0.3, anchor, depth, n, 3)
+    cs = tb.sample_function(lambda p: fields.gaussian(p, (0.55, 0.5, 0.45), 0.1), anchor, depth,
+                            n, 1)
+    fac = lambda t: math.cos(math.pi * t)
+    gv = tb.Grid(anchor, depth, n, 3, vs)
+    ov, oc = orc.Grid(anchor, depth, n, 3, vs), orc.Grid(anchor, depth, n, 1, cs)
+    dt = 0.02
+    pts = orc.grid_points(oc)
+    ref_dep = orc.grid_traj_rk2(ov, pts, dt, 0.0, 2, bc, g=fac)
+    if bc == 1:
+        ref_dep = wrap_periodic(ref_dep)
+    orc.grid_semilag_insitu(ov, oc, 1, dt, 2, bc, g=fac)
+    out = {}
+    try:
+        for lds in (2, 0):
+            _lib.set_option(_lib.OPT_GRID_LDS, lds)
+            for mode in ("exact", "fp64", "fp32"):
+                gc = tb.Grid(anchor, depth, n, 1, cs)
+                dep = torch.empty((len(pts), 3), dtype=torch.float64, device="cuda")
+                tb.grid_semilag_insitu(gv, gc, 1, dt, 2, bc, mode, g=fac, dep=dep)
+                out[(lds, mode)] = (dep.cpu().numpy(), gc.get_samples())
+    finally:
+        _lib.set_option(_lib.OPT_GRID_LDS, 1)
+    assert np.array_equal(out[(2, "exact")][0], ref_dep)
+    assert np.array_equal(out[(2, "exact")][1], oc.samples)
+    for mode in ("exact", "fp64", "fp32"):
+        assert np.array_equal(out[(2, mode)][0], out[(0, mode)][0]), mode
+        assert np.array_equal(out[(2, mode)][1], out[(0, mode)][1]), mode
