@@ -573,6 +573,72 @@ def batch_measure(steps: int = 2, nprob: int = 256):
     return out
 
 
+def combined_measure(steps: int = 2, nprob: int = 32):
+    """The many-tracer step and the persistent batch in one graph launch:
+    the 64 seeded tracers on the uniform depth-6, q = 8 tree (rotation
+    velocity, dt = 1/200, free space) attached to the batch of the first
+    nprob problems (workloads.batch_problems), FP32, so that each replay of
+    the batch's step graph advances both.  interps/step = N_tracers (2 + 64)
+    + 3 N_batch."""
+    import torch
+
+    import tbslas_amd as tb
+    from tbslas_amd import fields
+    from tbslas_amd.batch import Batch
+
+    q, T = 8, 64
+    anchor, depth = tb.uniform_leaves(6)
+    L, P = len(depth), (q + 1) ** 3
+    vel_t = tb.Tree(anchor, depth, q, 3, tb.fit_function(lambda p: fields.rotation(p, 2 * math.pi),
+                                                         anchor, depth, q, 3))
+    centres, sigma = fields.tracer_blobs(T, 3)
+    one = tb.Tree(anchor, depth, q, 1)
+    gp = one.grid_points()
+    C = torch.empty((L, T, tb.ncoef(q)), dtype=torch.float64, device="cuda")
+    ct, st = torch.from_numpy(centres).cuda(), torch.from_numpy(sigma).cuda()
+    for k in range(T):
+        one.set_grid_values(torch.exp(-((gp - ct[k]) ** 2).sum(1) / (2 * st[k] ** 2)))
+        C[:, k, :] = one.get_coeffs_device()[:, 0, :]
+    del gp, one
+    conc_t = tb.Tree(anchor, depth, q, T)
+    conc_t.set_coeffs(C)
+    del C
+    torch.cuda.empty_cache()
+    vel, conc, dts = [], [], []
+    npts = 0
+    for depth_u, qq, axis in W.batch_problems()[:nprob]:
+        a, d = tb.uniform_leaves(depth_u)
+        vel.append(tb.Tree(a, d, qq, 3, fields.rotation_coefficients(a, d, qq, axis, 2 * math.pi)))
+        c = tb.Tree(a, d, qq, 1)
+        g = c.grid_points()
+        c.set_grid_values(W.gaussian_grid_values(g))
+        del g
+        conc.append(c)
+        dts.append(1.0 / 200)
+        npts += len(d) * (qq + 1) ** 3
+    b = Batch(vel, conc, dts, 0)
+    b.tune()
+    b.attach_step(vel_t, conc_t, 1.0 / 200, 1, "fp32")
+    b.step(1)  # warm-up replay
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    b.step(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    interps = L * P * (2 + T) + 3 * npts
+    out = {"tracers": T, "tracer_leaves": L, "tracer_points": L * P, "problems": nprob,
+           "problem_points": npts, "interps_per_step": interps, "ms_per_step": ms,
+           "interps_per_s": interps / (ms * 1e-3), "dtype": "f32",
+           "graph": "one hipGraphLaunch per step: the tracer step's kernels, then the batch's "
+                    "counter reset and persistent kernel"}
+    b.close()
+    del b, vel, conc, vel_t, conc_t
+    torch.cuda.empty_cache()
+    return out
+
+
 def traffic_from_profiles(kernel: str, shape: int):
     """HBM bytes per launch of a kernel at a launch shape from the committed
     rocprofv3 counter summary (profiles/step_traffic.txt: "kernel shape
@@ -613,6 +679,9 @@ def main():
                          "(0: none; the first n of the 256)")
     ap.add_argument("--batch-only", action="store_true", help="only the 256-problem batch")
     ap.add_argument("--tracers-only", action="store_true", help="only the 64-tracer measurement")
+    ap.add_argument("--combined-only", action="store_true",
+                    help="only the 64 tracers and the first --batch-problems problems in one "
+                         "graph launch")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement")
     ap.add_argument("--tricubic-only", action="store_true", help="only the tricubic measurement")
     ap.add_argument("--uniform-only", action="store_true", help="only the uniform degree-8 case")
@@ -639,6 +708,10 @@ def main():
         return
     if args.tracers_only:
         print(json.dumps({"tracers_64": tracers_measure(2)}))
+        return
+    if args.combined_only:
+        print(json.dumps({"tracers_and_batch_one_graph": combined_measure(
+            2, max(1, args.batch_problems))}))
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint_20_steps": adjoint_measure()}))
