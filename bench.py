@@ -89,7 +89,23 @@ def run_mode(mode, vel, anchor, depth, ccf, steps, warmup):
     wall = time.perf_counter() - t0
     k_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
     r_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
-    return conc, wall, k_ms, r_ms
+    # the step's first launch, the velocity tree's expansion at its grid
+    # nodes (TBX_OPT_NODE_VELOCITY), timed alone after the timed steps so
+    # that the step kernel's own time is k_ms - x_ms
+    x_ms = 0.0
+    from tbslas_amd import _lib
+
+    if _lib.get_option(_lib.OPT_NODE_VELOCITY):
+        buf = torch.empty((vel.L, 3, vel.P), dtype=torch.float64, device="cuda")
+        xe = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
+        for e in xe:
+            e[0].record()
+            vel.grid_values(out=buf)
+            e[1].record()
+        torch.cuda.synchronize()
+        x_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in xe]))
+        del buf
+    return conc, wall, k_ms - x_ms, r_ms, x_ms
 
 
 def host_cpu() -> dict:
@@ -664,6 +680,8 @@ def main():
                          "traces (a seeded random sample)")
     ap.add_argument("--shape", type=int, default=None,
                     help="step kernel launch shape (default -1: per arithmetic mode)")
+    ap.add_argument("--node-velocity", type=int, choices=(0, 1), default=None,
+                    help="TBX_OPT_NODE_VELOCITY for every step (default: the library's, 1)")
     ap.add_argument("--profile", default=None, metavar="MODE",
                     help="counter-collection run: a 1 GiB device copy (the reference "
                          "for byte counters), then --steps step kernels of one "
@@ -702,6 +720,10 @@ def main():
     import torch
 
     import tbslas_amd as tb
+    from tbslas_amd import _lib
+
+    if args.node_velocity is not None:
+        _lib.set_option(_lib.OPT_NODE_VELOCITY, args.node_velocity)
 
     if args.batch_only:
         print(json.dumps({"batch_256_problems": batch_measure(2, 256)}))
@@ -778,7 +800,8 @@ def main():
     res = {}
     trees = {}
     for mode in ("fp64", "fp32", "fp32h"):
-        conc, wall, k_ms, r_ms = run_mode(mode, vel, anchor, depth, ccf, args.steps, args.warmup)
+        conc, wall, k_ms, r_ms, x_ms = run_mode(mode, vel, anchor, depth, ccf, args.steps,
+                                                args.warmup)
         trees[mode] = conc
         ms = 1e3 * wall / args.steps
         fl = flops_per_point(Q) * N
@@ -788,10 +811,12 @@ def main():
             "departure_points_per_s": N * args.steps / wall,
             "steps_per_s": args.steps / wall,
             "step_kernel_ms": k_ms,
+            "node_expansion_ms": x_ms,
             "refit_ms": r_ms,
             "step_kernel_tflops": fl / (k_ms * 1e-3) / 1e12,
         }
-        print(f"# {mode}: {ms:.2f} ms/step (step kernel {k_ms:.2f} ms, refit {r_ms:.2f} ms)",
+        print(f"# {mode}: {ms:.2f} ms/step (step kernel {k_ms:.2f} ms, node expansion "
+              f"{x_ms:.2f} ms, refit {r_ms:.2f} ms)",
               file=sys.stderr, flush=True)
 
     # each FP32 mode against FP64 after the timed steps (the FP64 path is
@@ -842,6 +867,8 @@ def main():
             "dt": DT, "bc": "periodic",
         },
         "headline_mode": "fp32h (FP32 contraction, degree <= 1 head in FP64)",
+        "first_stage": ("velocity expanded at the grid nodes each step (TBX_OPT_NODE_VELOCITY)"
+                        if _lib.get_option(_lib.OPT_NODE_VELOCITY) else "evaluated"),
         "fp64": res["fp64"],
         "fp32": res["fp32"],
         "fp32h": res["fp32h"],

@@ -173,6 +173,14 @@ int tbx_collect_grid_points(const tbx_tree *t, double *out, tbx_stream s);
  *      dof = 1 only).  coeff = vals * pinv(B)^T (cheb.h:166-196). -------- */
 int tbx_set_grid_values(tbx_tree *t, const double *vals, tbx_stream s);
 
+/* ---- The tree's values at its own grid points, vals [L][dof][P]: each
+ *      leaf's polynomial at its (q+1)^3 stretched nodes, upper-face nodes
+ *      included (the tensor-product Chebyshev values B c, by three one-axis
+ *      contractions).  No counterpart in the reference: it is the inverse of
+ *      tbx_set_grid_values on represented data, and the expansion a step's
+ *      first RK2 stage reads under TBX_OPT_NODE_VELOCITY. ------------------ */
+int tbx_tree_grid_values(const tbx_tree *t, double *vals, tbx_stream s);
+
 /* ---- SolveSemilagInSitu (tree/tree_semilag.h:92-135): one fused step.
  *      Arrival points are generated on the fly, RK2 runs through vel, conc
  *      is evaluated at the departure points and refit in place.  vel and
@@ -442,8 +450,27 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
  * TBX_OPT_GRID_LDS (default 1): the grid-leaf step on a uniform tree of 8^3-sample
  * leaves whose velocity and concentration trees share their leaves stages each
  * sibling block of 8 leaves in LDS, in FP32 steps (1) or in every mode (2); 0
- * gathers every stencil from global memory.  All give the same bits. */
-enum { TBX_OPT_STEP_SHAPE = 1, TBX_OPT_TRACER_MFMA = 2, TBX_OPT_GRID_LDS = 3 };
+ * gathers every stencil from global memory.  All give the same bits.
+ * TBX_OPT_NODE_VELOCITY (default 1): a step with a single velocity tree, in
+ * any mode but TBX_EXACT, expands the velocity tree at its own grid nodes
+ * (three one-axis contractions per leaf, into a buffer of L * 3 * (q+1)^3
+ * doubles the tree keeps) and takes the first RK2 stage's velocity at each
+ * arrival node from there when the node is a grid node of the velocity leaf
+ * it is located in (that leaf has the arrival leaf's depth); other nodes,
+ * and every later stage, are evaluated as before.  0 evaluates every stage.
+ * The two agree to rounding (the expansion's summation order; in the FP32
+ * modes the expansion is FP64).  The grid-leaf step (tbx_grid_semilag_insitu)
+ * takes the first stage at a sample point from the velocity's sample there
+ * under the same rule (velocity and concentration of the same N), the
+ * tricubic interpolation reproducing the sample up to rounding.  Steps
+ * captured into a batch graph keep the setting in force when the graph was
+ * made. */
+enum {
+  TBX_OPT_STEP_SHAPE = 1,
+  TBX_OPT_TRACER_MFMA = 2,
+  TBX_OPT_GRID_LDS = 3,
+  TBX_OPT_NODE_VELOCITY = 4
+};
 int tbx_set_option(int option, int value);
 int tbx_get_option(int option, int *value);
 

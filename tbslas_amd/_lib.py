@@ -24,6 +24,7 @@ MAX_DEPTH = 21
 OPT_STEP_SHAPE = 1
 OPT_TRACER_MFMA = 2
 OPT_GRID_LDS = 3
+OPT_NODE_VELOCITY = 4
 VEL_TREE = 0
 VEL_EXTRAP = 1
 VEL_SET = 2
@@ -92,6 +93,7 @@ _SIGS = {
                                _P, _P]),
     "tbx_collect_grid_points": (_I32, [_P, _P, _P]),
     "tbx_set_grid_values": (_I32, [_P, _P, _P]),
+    "tbx_tree_grid_values": (_I32, [_P, _P, _P]),
     "tbx_semilag_insitu": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P]),
     "tbx_semilag_values": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P, _P]),
     "tbx_semilag_insitu_noalloc": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P,

@@ -537,6 +537,12 @@ __device__ __forceinline__ void eval_velocity(const VelSource &vs, int stage, do
 // and ComputeTrajRK2 passes xsol as xinit); the midpoint is wrapped only
 // for its evaluation.
 template <int D, int MODE, int KIND>
+__device__ __forceinline__ void rk2_finish(const VelSource &vel, int bc, double htau,
+                                           double tau, int scaled, double g2, double ts2,
+                                           double &x, double &y, double &z, double v0,
+                                           double v1, double v2);
+
+template <int D, int MODE, int KIND>
 __device__ __forceinline__ void rk2_substep(const VelSource &vel, int bc,
                                             double htau, double tau, int scaled,
                                             double g1, double g2, double ts1,
@@ -549,6 +555,15 @@ __device__ __forceinline__ void rk2_substep(const VelSource &vel, int bc,
   }
   double v0, v1, v2;
   eval_velocity<D, MODE, KIND>(vel, 1, ts1, scaled, g1, bc, x, y, z, v0, v1, v2);
+  rk2_finish<D, MODE, KIND>(vel, bc, htau, tau, scaled, g2, ts2, x, y, z, v0, v1, v2);
+}
+
+// The substep's second stage, given the first stage's velocity at (x, y, z).
+template <int D, int MODE, int KIND>
+__device__ __forceinline__ void rk2_finish(const VelSource &vel, int bc, double htau,
+                                           double tau, int scaled, double g2, double ts2,
+                                           double &x, double &y, double &z, double v0,
+                                           double v1, double v2) {
   // xtmp = xinit + (0.5 * dt) * vtmp   (traj.inc:36)
   double hx = x + htau * v0, hy = y + htau * v1, hz = z + htau * v2;
   if (bc == TBX_BC_PERIODIC) {
@@ -561,6 +576,74 @@ __device__ __forceinline__ void rk2_substep(const VelSource &vel, int bc,
   x = x + tau * v0;
   y = y + tau * v1;
   z = z + tau * v2;
+}
+
+// ---------------------------------------------------------------------------
+// The first RK2 stage at an arrival grid node.
+//
+// A step's first substep starts at the arrival points, which are the grid
+// nodes corner + len * node[a] of the concentration's leaves; the first
+// stage evaluates the velocity there.  Where the velocity leaf that the
+// (wrapped) node is located in has the arrival leaf's depth, the node is one
+// of that leaf's own grid nodes, and its velocity is the velocity tree's
+// polynomial expanded at its nodes (tensor-product Chebyshev values, computed
+// once per step for all leaves) instead of a contraction.  That covers every
+// node of a leaf whose velocity leaf is the same cell, and the upper-face
+// nodes (node[D-1] = 1 up to rounding) that lie on the lower face of an
+// equal-sized neighbour (its node 0).
+// ---------------------------------------------------------------------------
+
+// Node index of arrival node m of a leaf `own` (D nodes per axis, the first
+// at the leaf's lower face and the last at its upper face up to rounding)
+// in the leaf `o` it is located in, or -1 when o is of another depth.  Per axis, the anchors at
+// the common depth differ by 0 (same node), +1 (own's upper face: o's node
+// 0) or -1 (own's lower face rounded below: o's node D-1), modulo the
+// periodic wrap.
+__device__ __forceinline__ int node_of(uint4 own, uint4 o, int m, int D) {
+  if (o.w != own.w) return -1;
+  const uint32_t mask = (1u << own.w) - 1u;
+  const uint32_t ov[3] = {o.x, o.y, o.z}, nv[3] = {own.x, own.y, own.z};
+  int r = 0, mul = 1, rest = m;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int ak = rest % D;
+    rest /= D;
+    const uint32_t d = (ov[k] - nv[k]) & mask;
+    int bk;
+    if (d == 0)
+      bk = ak;
+    else if (d == 1 && ak == D - 1)
+      bk = 0;
+    else if (d == mask && ak == 0)
+      bk = D - 1;
+    else
+      return -1;
+    r += bk * mul;
+    mul *= D;
+  }
+  return r;
+}
+
+// v = g * (velocity leaf lf's values at node node_of(own, lf, m)) from
+// vnode [L][3][P] (the scaling as in eval_velocity); false when m is not a
+// node of lf.
+template <int D>
+__device__ __forceinline__ bool nodal_velocity(const double *__restrict__ vnode,
+                                               const TreeView &vt, uint4 own, int m, int lf,
+                                               int P, int scaled, double g, double &v0,
+                                               double &v1, double &v2) {
+  const int mm = node_of(own, vt.info[lf], m, D);
+  if (mm < 0) return false;
+  const double *p = vnode + (int64_t)lf * 3 * P + mm;
+  v0 = p[0];
+  v1 = p[P];
+  v2 = p[2 * P];
+  if (scaled) {
+    v0 = v0 * g;
+    v1 = v1 * g;
+    v2 = v2 * g;
+  }
+  return true;
 }
 
 // ---------------------------------------------------------------------------

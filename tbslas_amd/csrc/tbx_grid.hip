@@ -153,6 +153,33 @@ __device__ __forceinline__ void grid_vel(const TreeView &t, const GridData &g, d
   }
 }
 
+// The first RK2 stage at sample point m of leaf `own`: where the velocity
+// leaf lf the point is located in has own's depth, the point is one of
+// lf's sample points (node_of) and its velocity is that sample (the
+// interpolation reproduces it up to rounding).  lf's samples start at
+// s + slot * 3 N^3 (slot = lf for the grid's own array, the leaf's place
+// for a staged copy); T: the sample precision the mode reads.  False when
+// the point is no sample of lf.
+template <typename T>
+__device__ __forceinline__ bool grid_node_velocity(const TreeView &tv, const T *__restrict__ s,
+                                                   int64_t slot, uint4 own, int m, int N, int lf,
+                                                   int scaled, double f, double &v0, double &v1,
+                                                   double &v2) {
+  const int mm = node_of(own, tv.info[lf], m, N);
+  if (mm < 0) return false;
+  const int64_t N3 = (int64_t)N * N * N;
+  const T *p = s + slot * 3 * N3 + mm;
+  v0 = (double)p[0];
+  v1 = (double)p[N3];
+  v2 = (double)p[2 * N3];
+  if (scaled) {
+    v0 = v0 * f;
+    v1 = v1 * f;
+    v2 = v2 * f;
+  }
+  return true;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_grid_eval(TreeView t, GridData g,
                                                       double *__restrict__ pts, int64_t n,
@@ -202,7 +229,17 @@ __global__ __launch_bounds__(kBlock) void k_grid_step(TreeView tv, GridData vel,
       z = wrap1(z);
     }
     double v0, v1, v2;
-    grid_vel<MODE>(tv, vel, x, y, z, a.scaled, a.g[2 * s], v0, v1, v2);
+    bool nodal = false;
+    if (s == 0 && a.nodal) {
+      const int lf = locate(tv, x, y, z);
+      if constexpr (MODE == kFp32)
+        nodal = grid_node_velocity(tv, vel.s32, lf, inf, (int)m, N, lf, a.scaled, a.g[0], v0, v1,
+                                   v2);
+      else
+        nodal = grid_node_velocity(tv, vel.s, lf, inf, (int)m, N, lf, a.scaled, a.g[0], v0, v1,
+                                   v2);
+    }
+    if (!nodal) grid_vel<MODE>(tv, vel, x, y, z, a.scaled, a.g[2 * s], v0, v1, v2);
     double hx = x + a.htau * v0, hy = y + a.htau * v1, hz = z + a.htau * v2;  // traj.inc:36
     if (a.bc == TBX_BC_PERIODIC) {
       hx = wrap1(hx);
@@ -261,6 +298,7 @@ __global__ __launch_bounds__(kBlkThreads) void k_grid_step_blk(TreeView tv, Grid
     gv = vel.s;
     gc = conc.s;
   }
+  const T *g0 = gv;  // the velocity samples of every leaf
   gv += L0 * 3 * kBlkN3;
   gc += L0 * kBlkN3;
   for (int i = tid; i < kBlkLeaves * 3 * kBlkN3; i += kBlkThreads) sv[i] = gv[i];
@@ -304,7 +342,17 @@ __global__ __launch_bounds__(kBlkThreads) void k_grid_step_blk(TreeView tv, Grid
         z = wrap1(z);
       }
       double v0, v1, v2;
-      velocity(x, y, z, a.g[2 * s], v0, v1, v2);
+      bool nodal = false;
+      if (s == 0 && a.nodal) {  // the sample point's own velocity sample
+        const int lf = locate(tv, x, y, z);
+        const int64_t lb = (int64_t)lf - L0;
+        if (lb >= 0 && lb < kBlkLeaves)
+          nodal = grid_node_velocity(tv, (const T *)sv, lb, inf, m, N, lf, a.scaled, a.g[0], v0,
+                                     v1, v2);
+        else
+          nodal = grid_node_velocity(tv, g0, lf, inf, m, N, lf, a.scaled, a.g[0], v0, v1, v2);
+      }
+      if (!nodal) velocity(x, y, z, a.g[2 * s], v0, v1, v2);
       double hx = x + a.htau * v0, hy = y + a.htau * v1, hz = z + a.htau * v2;  // traj.inc:36
       if (a.bc == TBX_BC_PERIODIC) {
         hx = wrap1(hx);

@@ -29,6 +29,7 @@ thread_local std::string g_err;
 std::atomic<int> g_step_shape{kAutoStepShape};
 std::atomic<int> g_tracer_mfma{1};
 std::atomic<int> g_grid_lds{1};
+std::atomic<int> g_node_velocity{1};
 
 int fail(int code, const char *fmt, ...) {
   char buf[512];
@@ -363,6 +364,7 @@ struct tbx_tree_s {
   int KP = 0, NP = 0;
   RefitTable tab{};
   double *d_adjA = nullptr;  // [L*P][3] per-point velocity cotangent factors
+  double *d_nodal = nullptr;  // [L][dof][P] values at the own grid nodes (velocity trees)
 
   TreeView view() const {
     TreeView v{};
@@ -396,6 +398,7 @@ void free_tree(tbx_tree *t) {
   hipFree(t->d_H);
   hipFree(t->d_W);
   hipFree(t->d_adjA);
+  hipFree(t->d_nodal);
   delete t;
 }
 
@@ -979,6 +982,55 @@ int tbx_set_grid_values(tbx_tree *t, const double *vals, tbx_stream s) {
 
 namespace {
 
+// vals [L][dof][P] = B c per leaf component (k_refit_expand, the refit's
+// transpose), the leaf's polynomial at its own grid nodes.
+int expand_grid_values(tbx_tree *t, double *vals, hipStream_t s, const char *who) {
+  int rc = ensure_refit_operator(t);
+  if (rc) return rc;
+  const int64_t rows = t->L * t->dof;
+  if (rows > INT32_MAX) return fail(TBX_EOVERFLOW, "%s: %lld rows", who, (long long)rows);
+  HIP_TRY(launch_refit_expand(t->q + 1, t->d_coeff, rows, t->tab, t->nc, vals, s));
+  return TBX_OK;
+}
+
+}  // namespace
+
+int tbx_tree_grid_values(const tbx_tree *t, double *vals, tbx_stream s) {
+  if (!t || !vals) return fail(TBX_EINVAL, "tbx_tree_grid_values: null argument");
+  return expand_grid_values(const_cast<tbx_tree *>(t), vals, (hipStream_t)s,
+                            "tbx_tree_grid_values");
+}
+
+namespace {
+
+// Whether a step's first stage reads the velocity tree's nodal values
+// (TBX_OPT_NODE_VELOCITY): single velocity trees, not in TBX_EXACT (which
+// evaluates every stage as the reference does, bit for bit).
+bool uses_node_velocity(int vel_kind, int mode) {
+  return g_node_velocity.load() && vel_kind == TBX_VEL_TREE && mode != kExact64;
+}
+
+size_t nodal_bytes(const tbx_tree *t) { return sizeof(double) * (size_t)t->L * t->dof * t->P(); }
+
+// The velocity tree's values at its own grid nodes, [L][3][P], expanded
+// from its coefficients by the three one-axis contractions (the refit's
+// transpose, k_refit_expand) into the tree's buffer or the arena.
+int node_velocity(tbx_tree *vt, hipStream_t s, ScratchArena *arena, const double **out,
+                  const char *who) {
+  double *nodal;
+  if (arena) {
+    nodal = (double *)arena->take(nodal_bytes(vt));
+    if (!nodal) return fail(TBX_EOVERFLOW, "%s: nodal velocities exceed the scratch", who);
+  } else {
+    if (!vt->d_nodal) HIP_TRY(hipMalloc(&vt->d_nodal, nodal_bytes(vt)));
+    nodal = vt->d_nodal;
+  }
+  int rc = expand_grid_values(vt, nodal, s, who);
+  if (rc) return rc;
+  *out = nodal;
+  return TBX_OK;
+}
+
 // Fused arrival -> RK2 -> conc evaluation into vals [L][dof][P].
 int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt,
                 int num_rk_step, int bc, uint32_t flags, double *vals, double *dep,
@@ -1027,6 +1079,10 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
     tcur = tcur + a.tau;
   }
   new_nodes_host(conc->q, a.node);
+  a.vnode = nullptr;
+  if (uses_node_velocity(vel->kind, mode) &&
+      (rc = node_velocity(const_cast<tbx_tree *>(vel->trees[0]), st, arena, &a.vnode, who)))
+    return rc;
   HIP_TRY(dispatch_step(conc->q, mode, vs, cv, a, vals, dep, st));
   return TBX_OK;
 }
@@ -1084,8 +1140,9 @@ int tbx_semilag_insitu_noalloc(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, t
   int mode, rc;
   if ((rc = mode_of(flags, &mode))) return rc;
   // the regions this call needs, each 256-byte aligned: values [L][dof][P],
-  // the refit workspace [L*dof][KP], and FP32 mirrors of trees that have
-  // none (FP32 modes)
+  // the refit workspace [L*dof][KP], FP32 mirrors of trees that have none
+  // (FP32 modes) and the velocity's nodal values [L][3][P] (every mode but
+  // TBX_EXACT, with TBX_OPT_NODE_VELOCITY on)
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t rows = (size_t)conc->L * conc->dof;
   size_t need = up(sizeof(double) * rows * conc->P()) + up(sizeof(double) * rows * conc->KP);
@@ -1093,6 +1150,7 @@ int tbx_semilag_insitu_noalloc(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, t
     if (!vel->d_coeff32) need += up(sizeof(float) * (size_t)vel->ncoeff_total() + kCoeffPad);
     if (!conc->d_coeff32) need += up(sizeof(float) * (size_t)conc->ncoeff_total() + kCoeffPad);
   }
+  if (uses_node_velocity(TBX_VEL_TREE, mode)) need += up(nodal_bytes(vel));
   if (scratch_bytes) *scratch_bytes = need;
   void *base = nullptr;
   size_t cap = 0;
@@ -1546,6 +1604,9 @@ int tbx_grid_semilag_insitu(const tbx_grid *velc, tbx_tfac_fn g, void *gctx, tbx
     for (int64_t j = 0; blk && j < tc->L; ++j) blk = (int)tc->h_info[j].w == tc->max_depth;
     a.blk = blk ? 1 : 0;
   }
+  // the first stage at a sample point from the velocity sample there
+  // (TBX_OPT_NODE_VELOCITY; sample points of equal-sized leaves coincide)
+  a.nodal = uses_node_velocity(TBX_VEL_TREE, mode) && vel->n == N ? 1 : 0;
   HIP_TRY(launch_grid_step(mode, vel->tree->view(), vel->d_s, vel->d_s32, vel->n,
                            conc->tree->view(), conc->d_s, conc->d_s32, N, conc->dof, a,
                            conc->d_s_next, dep, st));
@@ -1624,6 +1685,18 @@ int batch_configure(tbx_batch *b, int cfg) {
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   b->grid = ncu * batch_blocks_per_cu(cfg);
+  // the attached steps' nodal velocity buffers (the options in force now
+  // decide whether the captured steps use them)
+  for (const auto &at : b->attached) {
+    int md;
+    if (mode_of(at.flags, &md) == TBX_OK && uses_node_velocity(TBX_VEL_TREE, md) &&
+        !at.vel->d_nodal) {
+      tbx_tree *vt = const_cast<tbx_tree *>(at.vel);
+      int rc = ensure_refit_operator(vt);
+      if (rc) return rc;
+      HIP_TRY(hipMalloc(&vt->d_nodal, nodal_bytes(vt)));
+    }
+  }
   for (int k = 0; k < 2; ++k) {
     HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
     // the attached steps' kernels (their buffers were made when attached,
@@ -1862,6 +1935,12 @@ int tbx_set_option(int option, int value) {
     g_grid_lds = value;
     return TBX_OK;
   }
+  if (option == TBX_OPT_NODE_VELOCITY) {
+    if (value != 0 && value != 1)
+      return fail(TBX_EINVAL, "tbx_set_option: node velocity switch %d is not 0 or 1", value);
+    g_node_velocity = value;
+    return TBX_OK;
+  }
   return fail(TBX_EINVAL, "tbx_set_option: unknown option %d", option);
 }
 
@@ -1886,6 +1965,10 @@ int tbx_get_option(int option, int *value) {
   }
   if (option == TBX_OPT_GRID_LDS) {
     *value = g_grid_lds;
+    return TBX_OK;
+  }
+  if (option == TBX_OPT_NODE_VELOCITY) {
+    *value = g_node_velocity;
     return TBX_OK;
   }
   return fail(TBX_EINVAL, "tbx_get_option: unknown option %d", option);
@@ -2325,10 +2408,12 @@ int rebuild(tbx_tree *t, const std::vector<NewLeaf> &leaves, const double *pcoef
   (void)hipFree(t->d_vals);
   (void)hipFree(t->d_W);
   (void)hipFree(t->d_adjA);
+  (void)hipFree(t->d_nodal);
   t->d_coeff32 = nullptr;
   t->d_vals = nullptr;
   t->d_W = nullptr;
   t->d_adjA = nullptr;
+  t->d_nodal = nullptr;
   t->f32_stale = true;
   return TBX_OK;
 }

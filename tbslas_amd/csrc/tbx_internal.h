@@ -116,6 +116,9 @@ struct StepArgs {
   double g[2 * kMaxSub];
   double ts[2 * kMaxSub];  // stage times
   double node[TBX_MAX_Q + 1];  // stretched nodes (cheb.h:53-58)
+  // the velocity tree's values at its own grid nodes, [L][3][P] (null: the
+  // first stage evaluates like the second); single velocity trees only
+  const double *vnode;
 };
 
 constexpr int kBlock = 256;
@@ -171,6 +174,8 @@ struct GridStepArgs {
   int32_t scaled;
   int32_t blk;   // uniform tree of 8^3-sample leaves shared by velocity and
                  // concentration: sibling blocks staged in LDS
+  int32_t nodal;  // first RK2 stage at a sample point from the velocity's
+                  // sample there (velocity and concentration of one N)
   double tau, htau;
   double g[2 * kMaxSub];
   double node[16];  // a / (N - 1)

@@ -114,7 +114,25 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
   double y = (double)inf.y * len + len * a.node[(m / D) % D];
   double z = (double)inf.z * len + len * a.node[m / (D * D)];
   // ComputeTrajRK2 (traj.inc:48-68), tinit -> tfinal = tinit - dt
-  for (int s = 0; s < a.nsub; ++s)
+  int s = 0;
+  if constexpr (KIND == kVelTree) {
+    if (a.vnode) {  // the first stage at the arrival node from the nodal values
+      if (a.bc == TBX_BC_PERIODIC) {
+        x = wrap1(x);
+        y = wrap1(y);
+        z = wrap1(z);
+      }
+      const int lf = locate(vel.t[0], x, y, z);
+      double v0, v1, v2;
+      if (!nodal_velocity<D>(a.vnode, vel.t[0], inf, m, lf, a.P, a.scaled, a.g[0], v0, v1, v2))
+        eval_velocity<D, MODE, KIND>(vel, 1, a.ts[0], a.scaled, a.g[0], a.bc, x, y, z, v0, v1,
+                                     v2, lf);
+      rk2_finish<D, MODE, KIND>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[1], a.ts[1], x, y, z,
+                                v0, v1, v2);
+      s = 1;
+    }
+  }
+  for (; s < a.nsub; ++s)
     rk2_substep<D, MODE, KIND>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[2 * s],
                          a.g[2 * s + 1], a.ts[2 * s], a.ts[2 * s + 1], x, y, z);
   // con_evaluator(departure points) wraps them in place (semilag.inc:43)
@@ -146,6 +164,23 @@ __device__ __forceinline__ void reload_start(RegroupLds<BLOCK, NF> &sh, double (
   st[0] = sh.f[0][threadIdx.x];
   st[1] = sh.f[1][threadIdx.x];
   st[2] = sh.f[2][threadIdx.x];
+}
+
+// Leaf and node of arrival point i0 + t of a workgroup starting at point i0
+// (t < the workgroup size), points past the last taken as the last: the
+// workgroup-uniform i0 / P in scalar registers, the rest in 32 bits.
+__device__ __forceinline__ void arrival_node(const StepArgs &a, int64_t i0, int t,
+                                             int64_t &leaf, int &m) {
+  if (i0 + t >= a.npts) {
+    leaf = a.npts / a.P - 1;
+    m = a.P - 1;
+    return;
+  }
+  const int64_t l0 = i0 / a.P;
+  const uint32_t r = (uint32_t)(i0 - l0 * a.P) + (uint32_t)t;
+  const uint32_t q = r / (uint32_t)a.P;
+  leaf = l0 + q;
+  m = (int)(r - q * (uint32_t)a.P);
 }
 
 // The same step with the workgroup's points regrouped by leaf before each
@@ -184,17 +219,46 @@ __device__ __forceinline__ void k_step_rg_body(const VelSource &vel, const TreeV
         st[2] = wrap1(st[2]);
       }
       lf = locate(vel.t[0], st[0], st[1], st[2]);
-      {
-        double p3[3] = {st[0], st[1], st[2]};
-        regroup(sh, orig, lf, p3);
-        st[0] = p3[0];
-        st[1] = p3[1];
-        st[2] = p3[2];
-      }
       double v0, v1, v2;
-      eval_velocity<D, MODE, KIND>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, st[0], st[1],
-                             st[2], v0, v1, v2, lf);
-      reload_start(sh, st);
+      bool nodal = false;
+      if constexpr (KIND == kVelTree) {
+        // the first substep starts at the arrival nodes: their velocities
+        // from the nodal values, and the regroup and contraction only in
+        // workgroups holding a node of another depth's leaf (whose points
+        // then take the nodal value again where they have one, so that a
+        // point's result does not depend on the workgroup it falls in)
+        if (s == 0 && a.vnode) {
+          // (the arrival node recomputed from the thread index: nothing of it
+          // stays live through the substeps)
+          int64_t jl;
+          int jm;
+          arrival_node(a, i0, (int)threadIdx.x, jl, jm);
+          const bool got = nodal_velocity<D>(a.vnode, vel.t[0], conc.info[jl], jm, lf, a.P,
+                                             a.scaled, a.g[0], v0, v1, v2);
+          nodal = !__syncthreads_or(!got);
+        }
+      }
+      if (!nodal) {
+        {
+          double p3[3] = {st[0], st[1], st[2]};
+          regroup(sh, orig, lf, p3);
+          st[0] = p3[0];
+          st[1] = p3[1];
+          st[2] = p3[2];
+        }
+        eval_velocity<D, MODE, KIND>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, st[0],
+                                     st[1], st[2], v0, v1, v2, lf);
+        reload_start(sh, st);
+        if constexpr (KIND == kVelTree) {
+          if (s == 0 && a.vnode) {  // the point this thread now holds
+            int64_t jl;
+            int jm;
+            arrival_node(a, i0, orig, jl, jm);
+            nodal_velocity<D>(a.vnode, vel.t[0], conc.info[jl], jm, lf, a.P, a.scaled, a.g[0],
+                              v0, v1, v2);
+          }
+        }
+      }
       // midpoint xtmp = xinit + (0.5 * dt) * vtmp (traj.inc:36), wrapped
       // for its evaluation only
       st[3] = st[0] + a.htau * v0;

@@ -284,6 +284,19 @@ class Tree:
         values on order^3 uniform points per leaf (VTK hexahedra)."""
         check(lib().tbx_tree_write_vtu(self._h, prefix.encode(), order, _stream()))
 
+    def grid_values(self, out=None):
+        """The tree's values at its own grid points, [L][dof][P] (each leaf's
+        polynomial at its nodes, upper faces included)."""
+        import torch
+
+        if out is None:
+            out = torch.empty((self.L, self.dof, self.P), dtype=torch.float64, device="cuda")
+        _require_dev(out, torch.float64, "out")
+        if out.numel() != self.L * self.dof * self.P:
+            raise ValueError("out must hold L*dof*P values")
+        check(lib().tbx_tree_grid_values(self._h, _ptr(out), _stream()))
+        return out
+
     def set_grid_values(self, vals):
         """SetTreeGridValues with vals [L][dof][P] (dof-major per leaf)."""
         import torch

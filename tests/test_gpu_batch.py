@@ -93,12 +93,21 @@ def test_batch_matches_per_problem_steps(torch, tb):
     # every step the same map).  Against FP64 the difference would be the
     # face flips of FP32 departure points on these coarse problems (leaf
     # faces the rotation axis crosses, fit jumps ~1e-4), not rounding.
+    # The batch evaluates every RK2 stage, so the references do too
+    # (TBX_OPT_NODE_VELOCITY off: no first stage from nodal values).
+    from tbslas_amd import _lib
+
     refs = []
-    for (a, d, q, v, c, dt), tv in zip(probs, vel):
-        t = tb.Tree(a, d, q, 1, c)
-        for _ in range(3):
-            tb.semilag_insitu(tv, t, 1, dt, 1, 0, "fp32")
-        refs.append(t.get_coeffs())
+    old = _lib.get_option(_lib.OPT_NODE_VELOCITY)
+    _lib.set_option(_lib.OPT_NODE_VELOCITY, 0)
+    try:
+        for (a, d, q, v, c, dt), tv in zip(probs, vel):
+            t = tb.Tree(a, d, q, 1, c)
+            for _ in range(3):
+                tb.semilag_insitu(tv, t, 1, dt, 1, 0, "fp32")
+            refs.append(t.get_coeffs())
+    finally:
+        _lib.set_option(_lib.OPT_NODE_VELOCITY, old)
     results = []
     for cfg in range(len(CONFIGS)):
         b = Batch(vel, conc, dts, 0)
