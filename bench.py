@@ -57,6 +57,26 @@ def flops_per_point(q: int) -> int:
     return 2 * (9 * d + 2 * nc * 3) + (9 * d + 2 * nc * 1) + 18
 
 
+def executed_flops_per_point(q: int) -> int:
+    """Flops the step kernel itself executes per departure point when the
+    first RK2 stage reads the velocity's nodal values (TBX_OPT_NODE_VELOCITY):
+    the reference's count less one velocity evaluation (the few nodes on a
+    face towards a leaf of another depth, 0.2 % on the benchmark tree, still
+    evaluate and are not counted)."""
+    d = q + 1
+    nc = d * (d + 1) * (d + 2) // 6
+    return flops_per_point(q) - (9 * d + 2 * nc * 3)
+
+
+def expansion_flops_per_point(q: int) -> float:
+    """Flops k_refit_expand executes per grid point for the velocity's
+    expansion at its nodes: per leaf component D^3 (D+1)/2 multiply-adds for
+    the z contraction over the (j, k) triangle and D^4 each for y and x;
+    3 components, D^3 points per leaf."""
+    d = q + 1
+    return 2.0 * (d ** 3 * (d + 1) / 2 + 2 * d ** 4) * 3 / d ** 3
+
+
 def build_workload():
     return W.adaptive_q14()
 
@@ -96,7 +116,9 @@ def run_mode(mode, vel, anchor, depth, ccf, steps, warmup):
     from tbslas_amd import _lib
 
     if _lib.get_option(_lib.OPT_NODE_VELOCITY):
-        buf = torch.empty((vel.L, 3, vel.P), dtype=torch.float64, device="cuda")
+        # FP32 nodal values in the FP32 modes, as the step writes them
+        buf = torch.empty((vel.L, 3, vel.P), device="cuda",
+                          dtype=torch.float64 if mode == "fp64" else torch.float32)
         xe = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
         for e in xe:
             e[0].record()
@@ -844,6 +866,22 @@ def main():
             r["peak_note"] = ("published FP32 vector peak, reached only by packed FP32, which "
                               "this build does not issue")
             r["frac_of_unpacked_fp32"] = ach / PEAK_FP32_UNPACKED
+        if _lib.get_option(_lib.OPT_NODE_VELOCITY):
+            # achieved counts the reference's flops; the kernel executes fewer
+            ex = executed_flops_per_point(Q) * N / k_s / 1e12
+            r["executed_flops_per_point"] = executed_flops_per_point(Q)
+            r["executed_tflops"] = ex
+            r["executed_frac"] = ex / peak
+            x_s = res[mode]["node_expansion_ms"] * 1e-3
+            r["node_expansion"] = {
+                "kernel": "k_refit_expand", "ms": res[mode]["node_expansion_ms"],
+                "flops_per_point": expansion_flops_per_point(Q),
+                "tflops": expansion_flops_per_point(Q) * N / x_s / 1e12 if x_s > 0 else None,
+                "bytes": L * 3 * (8 * ((Q + 1) * (Q + 2) * (Q + 3) // 6)
+                                  + (8 if mode == "fp64" else 4) * P),
+            }
+            if x_s > 0:
+                r["node_expansion"]["hbm_gbs"] = r["node_expansion"]["bytes"] / x_s / 1e9
         return r
 
     out = {

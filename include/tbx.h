@@ -180,6 +180,8 @@ int tbx_set_grid_values(tbx_tree *t, const double *vals, tbx_stream s);
  *      tbx_set_grid_values on represented data, and the expansion a step's
  *      first RK2 stage reads under TBX_OPT_NODE_VELOCITY. ------------------ */
 int tbx_tree_grid_values(const tbx_tree *t, double *vals, tbx_stream s);
+/* The same values rounded to FP32 (what the FP32 modes' first stage reads). */
+int tbx_tree_grid_values_f32(const tbx_tree *t, float *vals, tbx_stream s);
 
 /* ---- SolveSemilagInSitu (tree/tree_semilag.h:92-135): one fused step.
  *      Arrival points are generated on the fly, RK2 runs through vel, conc

@@ -94,6 +94,7 @@ _SIGS = {
     "tbx_collect_grid_points": (_I32, [_P, _P, _P]),
     "tbx_set_grid_values": (_I32, [_P, _P, _P]),
     "tbx_tree_grid_values": (_I32, [_P, _P, _P]),
+    "tbx_tree_grid_values_f32": (_I32, [_P, _P, _P]),
     "tbx_semilag_insitu": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P]),
     "tbx_semilag_values": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P, _P, _P]),
     "tbx_semilag_insitu_noalloc": (_I32, [_P, TFAC_FN, _P, _P, _I32, _D, _I32, _I32, _U32, _P,

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "tbx_internal.h"
 
 namespace tbx {
@@ -593,6 +595,10 @@ __device__ __forceinline__ void rk2_finish(const VelSource &vel, int bc, double 
 // equal-sized neighbour (its node 0).
 // ---------------------------------------------------------------------------
 
+// The precision of the nodal velocities a step of arithmetic mode MODE reads.
+template <int MODE>
+using NodalT = typename std::conditional<MODE == kFp32 || MODE == kFp32h, float, double>::type;
+
 // Node index of arrival node m of a leaf `own` (D nodes per axis, the first
 // at the leaf's lower face and the last at its upper face up to rounding)
 // in the leaf `o` it is located in, or -1 when o is of another depth.  Per axis, the anchors at
@@ -627,17 +633,17 @@ __device__ __forceinline__ int node_of(uint4 own, uint4 o, int m, int D) {
 // v = g * (velocity leaf lf's values at node node_of(own, lf, m)) from
 // vnode [L][3][P] (the scaling as in eval_velocity); false when m is not a
 // node of lf.
-template <int D>
-__device__ __forceinline__ bool nodal_velocity(const double *__restrict__ vnode,
+template <int D, typename VT>
+__device__ __forceinline__ bool nodal_velocity(const VT *__restrict__ vnode,
                                                const TreeView &vt, uint4 own, int m, int lf,
                                                int P, int scaled, double g, double &v0,
                                                double &v1, double &v2) {
   const int mm = node_of(own, vt.info[lf], m, D);
   if (mm < 0) return false;
-  const double *p = vnode + (int64_t)lf * 3 * P + mm;
-  v0 = p[0];
-  v1 = p[P];
-  v2 = p[2 * P];
+  const VT *p = vnode + (int64_t)lf * 3 * P + mm;
+  v0 = (double)p[0];
+  v1 = (double)p[P];
+  v2 = (double)p[2 * P];
   if (scaled) {
     v0 = v0 * g;
     v1 = v1 * g;

@@ -983,13 +983,18 @@ int tbx_set_grid_values(tbx_tree *t, const double *vals, tbx_stream s) {
 namespace {
 
 // vals [L][dof][P] = B c per leaf component (k_refit_expand, the refit's
-// transpose), the leaf's polynomial at its own grid nodes.
-int expand_grid_values(tbx_tree *t, double *vals, hipStream_t s, const char *who) {
+// transpose), the leaf's polynomial at its own grid nodes, into vals (FP64)
+// or vals32 (rounded to FP32).
+int expand_grid_values(tbx_tree *t, double *vals, float *vals32, hipStream_t s,
+                       const char *who) {
   int rc = ensure_refit_operator(t);
   if (rc) return rc;
   const int64_t rows = t->L * t->dof;
   if (rows > INT32_MAX) return fail(TBX_EOVERFLOW, "%s: %lld rows", who, (long long)rows);
-  HIP_TRY(launch_refit_expand(t->q + 1, t->d_coeff, rows, t->tab, t->nc, vals, s));
+  if (vals32)
+    HIP_TRY(launch_refit_expand32(t->q + 1, t->d_coeff, rows, t->tab, t->nc, vals32, s));
+  else
+    HIP_TRY(launch_refit_expand(t->q + 1, t->d_coeff, rows, t->tab, t->nc, vals, s));
   return TBX_OK;
 }
 
@@ -997,8 +1002,14 @@ int expand_grid_values(tbx_tree *t, double *vals, hipStream_t s, const char *who
 
 int tbx_tree_grid_values(const tbx_tree *t, double *vals, tbx_stream s) {
   if (!t || !vals) return fail(TBX_EINVAL, "tbx_tree_grid_values: null argument");
-  return expand_grid_values(const_cast<tbx_tree *>(t), vals, (hipStream_t)s,
+  return expand_grid_values(const_cast<tbx_tree *>(t), vals, nullptr, (hipStream_t)s,
                             "tbx_tree_grid_values");
+}
+
+int tbx_tree_grid_values_f32(const tbx_tree *t, float *vals, tbx_stream s) {
+  if (!t || !vals) return fail(TBX_EINVAL, "tbx_tree_grid_values_f32: null argument");
+  return expand_grid_values(const_cast<tbx_tree *>(t), nullptr, vals, (hipStream_t)s,
+                            "tbx_tree_grid_values_f32");
 }
 
 namespace {
@@ -1010,22 +1021,30 @@ bool uses_node_velocity(int vel_kind, int mode) {
   return g_node_velocity.load() && vel_kind == TBX_VEL_TREE && mode != kExact64;
 }
 
-size_t nodal_bytes(const tbx_tree *t) { return sizeof(double) * (size_t)t->L * t->dof * t->P(); }
+// Bytes of a tree's nodal values in a mode (FP32 in the FP32 modes); the
+// tree's own buffer is sized for FP64.
+size_t nodal_bytes(const tbx_tree *t, int mode = kFma64) {
+  const size_t w = (mode == kFp32 || mode == kFp32h) ? sizeof(float) : sizeof(double);
+  return w * (size_t)t->L * t->dof * t->P();
+}
 
 // The velocity tree's values at its own grid nodes, [L][3][P], expanded
 // from its coefficients by the three one-axis contractions (the refit's
-// transpose, k_refit_expand) into the tree's buffer or the arena.
-int node_velocity(tbx_tree *vt, hipStream_t s, ScratchArena *arena, const double **out,
+// transpose, k_refit_expand; rounded to FP32 in the FP32 modes) into the
+// tree's buffer or the arena.
+int node_velocity(tbx_tree *vt, int mode, hipStream_t s, ScratchArena *arena, const void **out,
                   const char *who) {
-  double *nodal;
+  void *nodal;
   if (arena) {
-    nodal = (double *)arena->take(nodal_bytes(vt));
+    nodal = arena->take(nodal_bytes(vt, mode));
     if (!nodal) return fail(TBX_EOVERFLOW, "%s: nodal velocities exceed the scratch", who);
   } else {
     if (!vt->d_nodal) HIP_TRY(hipMalloc(&vt->d_nodal, nodal_bytes(vt)));
     nodal = vt->d_nodal;
   }
-  int rc = expand_grid_values(vt, nodal, s, who);
+  const bool f32 = mode == kFp32 || mode == kFp32h;
+  int rc = expand_grid_values(vt, f32 ? nullptr : (double *)nodal, f32 ? (float *)nodal : nullptr,
+                              s, who);
   if (rc) return rc;
   *out = nodal;
   return TBX_OK;
@@ -1081,7 +1100,8 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
   new_nodes_host(conc->q, a.node);
   a.vnode = nullptr;
   if (uses_node_velocity(vel->kind, mode) &&
-      (rc = node_velocity(const_cast<tbx_tree *>(vel->trees[0]), st, arena, &a.vnode, who)))
+      (rc = node_velocity(const_cast<tbx_tree *>(vel->trees[0]), mode, st, arena, &a.vnode,
+                          who)))
     return rc;
   HIP_TRY(dispatch_step(conc->q, mode, vs, cv, a, vals, dep, st));
   return TBX_OK;
@@ -1150,7 +1170,7 @@ int tbx_semilag_insitu_noalloc(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, t
     if (!vel->d_coeff32) need += up(sizeof(float) * (size_t)vel->ncoeff_total() + kCoeffPad);
     if (!conc->d_coeff32) need += up(sizeof(float) * (size_t)conc->ncoeff_total() + kCoeffPad);
   }
-  if (uses_node_velocity(TBX_VEL_TREE, mode)) need += up(nodal_bytes(vel));
+  if (uses_node_velocity(TBX_VEL_TREE, mode)) need += up(nodal_bytes(vel, mode));
   if (scratch_bytes) *scratch_bytes = need;
   void *base = nullptr;
   size_t cap = 0;

@@ -116,9 +116,10 @@ struct StepArgs {
   double g[2 * kMaxSub];
   double ts[2 * kMaxSub];  // stage times
   double node[TBX_MAX_Q + 1];  // stretched nodes (cheb.h:53-58)
-  // the velocity tree's values at its own grid nodes, [L][3][P] (null: the
-  // first stage evaluates like the second); single velocity trees only
-  const double *vnode;
+  // the velocity tree's values at its own grid nodes, [L][3][P], FP32 in the
+  // FP32 modes and FP64 otherwise (null: the first stage evaluates like the
+  // second); single velocity trees only
+  const void *vnode;
 };
 
 constexpr int kBlock = 256;
@@ -232,6 +233,9 @@ hipError_t launch_refit_project(int d, const double *V, int64_t rows, int dof, i
 // V = B U per row: U [rows][KP] (packed triangle), V [rows][d^3].
 hipError_t launch_refit_expand(int d, const double *U, int64_t rows, const RefitTable &tab,
                                int KP, double *V, hipStream_t s);
+// The same with V rounded to FP32.
+hipError_t launch_refit_expand32(int d, const double *U, int64_t rows, const RefitTable &tab,
+                                 int KP, float *V, hipStream_t s);
 // CheckChebOutput's block sums (metric_blocks(npts) blocks; pass 0: dof
 // sums per block, pass 1: 4 values per block); dof <= 8.
 int metric_blocks(int64_t npts);

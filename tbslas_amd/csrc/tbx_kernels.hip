@@ -124,7 +124,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
       }
       const int lf = locate(vel.t[0], x, y, z);
       double v0, v1, v2;
-      if (!nodal_velocity<D>(a.vnode, vel.t[0], inf, m, lf, a.P, a.scaled, a.g[0], v0, v1, v2))
+      if (!nodal_velocity<D>((const NodalT<MODE> *)a.vnode, vel.t[0], inf, m, lf, a.P, a.scaled, a.g[0], v0, v1, v2))
         eval_velocity<D, MODE, KIND>(vel, 1, a.ts[0], a.scaled, a.g[0], a.bc, x, y, z, v0, v1,
                                      v2, lf);
       rk2_finish<D, MODE, KIND>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[1], a.ts[1], x, y, z,
@@ -233,7 +233,7 @@ __device__ __forceinline__ void k_step_rg_body(const VelSource &vel, const TreeV
           int64_t jl;
           int jm;
           arrival_node(a, i0, (int)threadIdx.x, jl, jm);
-          const bool got = nodal_velocity<D>(a.vnode, vel.t[0], conc.info[jl], jm, lf, a.P,
+          const bool got = nodal_velocity<D>((const NodalT<MODE> *)a.vnode, vel.t[0], conc.info[jl], jm, lf, a.P,
                                              a.scaled, a.g[0], v0, v1, v2);
           nodal = !__syncthreads_or(!got);
         }
@@ -254,7 +254,7 @@ __device__ __forceinline__ void k_step_rg_body(const VelSource &vel, const TreeV
             int64_t jl;
             int jm;
             arrival_node(a, i0, orig, jl, jm);
-            nodal_velocity<D>(a.vnode, vel.t[0], conc.info[jl], jm, lf, a.P, a.scaled, a.g[0],
+            nodal_velocity<D>((const NodalT<MODE> *)a.vnode, vel.t[0], conc.info[jl], jm, lf, a.P, a.scaled, a.g[0],
                               v0, v1, v2);
           }
         }

@@ -216,11 +216,17 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
   }
 }
 
-// The transpose: v = B u per row (u rows of stride KP in packed order).
-template <int D>
+// The transpose: v = B u per row (u rows of stride KP in packed order), the
+// row's polynomial at the D^3 grid nodes, v[c][b][a] (x fastest), by the
+// three one-axis contractions z, y, x through LDS.  TO: the output precision
+// (FP32 for the FP32 modes' nodal velocities).  (A variant contracting x
+// first over the triangle, with y and z in registers, does a third fewer
+// multiply-adds but ran 1.3x slower: its scalar-operand and register loads
+// cost more than the LDS reads they replace.)
+template <int D, typename TO>
 __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__restrict__ U,
                                                                 int64_t rows, RefitTable tab,
-                                                                int KP, double *__restrict__ V) {
+                                                                int KP, TO *__restrict__ V) {
   constexpr int P = D * D * D;
   __shared__ double sa[P];
   __shared__ double sb[P];
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__
     }
   }
   __syncthreads();
-  double *v = V + r * P;
+  TO *v = V + r * P;
   for (int t = tid; t < D * D; t += kRefitThreads) {  // t = c * D + b
     double x[D];
 #pragma unroll
@@ -268,7 +274,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__
       double s = 0.0;
 #pragma unroll
       for (int k = 0; k < D; ++k) s = __builtin_fma(sT[k * D + a], x[k], s);
-      v[t * D + a] = s;
+      v[t * D + a] = (TO)s;
     }
   }
   __syncthreads();  // the next row reuses sa and sb
@@ -423,15 +429,19 @@ hipError_t launch_refit_project(int d, const double *V, int64_t rows, int dof, i
   return hipGetLastError();
 }
 
-hipError_t launch_refit_expand(int d, const double *U, int64_t rows, const RefitTable &tab,
-                               int KP, double *V, hipStream_t s) {
+template <typename TO>
+hipError_t launch_expand(int d, const double *U, int64_t rows, const RefitTable &tab, int KP,
+                         TO *V, hipStream_t s) {
   if (rows <= 0) return hipSuccess;
-  const unsigned nb = (unsigned)(rows < kRowBlocks ? rows : kRowBlocks);
+  // 8,192 row-strided workgroups: 9 % faster than one per row on 108,972
+  // rows of degree 14 (the table set-up is paid once per workgroup)
+  constexpr int64_t kExpandBlocks = 8192;
+  const unsigned nb = (unsigned)(rows < kExpandBlocks ? rows : kExpandBlocks);
   switch (d) {
 #define X(D)                                                                             \
   case D:                                                                                \
-    hipLaunchKernelGGL(k_refit_expand<D>, dim3(nb), dim3(kRefitThreads), 0, s, U, rows, tab, \
-                       KP, V);                                                           \
+    hipLaunchKernelGGL((k_refit_expand<D, TO>), dim3(nb), dim3(kRefitThreads), 0, s, U, rows, \
+                       tab, KP, V);                                                      \
     break;
     TBX_REFIT_CASES(X)
 #undef X
@@ -439,6 +449,16 @@ hipError_t launch_refit_expand(int d, const double *U, int64_t rows, const Refit
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_refit_expand(int d, const double *U, int64_t rows, const RefitTable &tab,
+                               int KP, double *V, hipStream_t s) {
+  return launch_expand<double>(d, U, rows, tab, KP, V, s);
+}
+
+hipError_t launch_refit_expand32(int d, const double *U, int64_t rows, const RefitTable &tab,
+                                 int KP, float *V, hipStream_t s) {
+  return launch_expand<float>(d, U, rows, tab, KP, V, s);
 }
 
 int diff_norm2_blocks(int64_t n) {

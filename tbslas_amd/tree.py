@@ -286,15 +286,18 @@ class Tree:
 
     def grid_values(self, out=None):
         """The tree's values at its own grid points, [L][dof][P] (each leaf's
-        polynomial at its nodes, upper faces included)."""
+        polynomial at its nodes, upper faces included); a float32 out
+        receives them rounded to FP32."""
         import torch
 
         if out is None:
             out = torch.empty((self.L, self.dof, self.P), dtype=torch.float64, device="cuda")
-        _require_dev(out, torch.float64, "out")
+        f32 = out.dtype == torch.float32
+        _require_dev(out, torch.float32 if f32 else torch.float64, "out")
         if out.numel() != self.L * self.dof * self.P:
             raise ValueError("out must hold L*dof*P values")
-        check(lib().tbx_tree_grid_values(self._h, _ptr(out), _stream()))
+        fn = lib().tbx_tree_grid_values_f32 if f32 else lib().tbx_tree_grid_values
+        check(fn(self._h, _ptr(out), _stream()))
         return out
 
     def set_grid_values(self, vals):

@@ -198,6 +198,9 @@ def test_grid_values_against_oracle(torch, tb, orc, q, dof):
     got = t.grid_values().cpu().numpy()
     assert got.shape == ref.shape
     assert rel(got, ref) <= 1e-13, rel(got, ref)
+    # the FP32 variant: the same values rounded once
+    g32 = t.grid_values(torch.empty((t.L, dof, P), dtype=torch.float32, device="cuda"))
+    assert np.array_equal(g32.cpu().numpy(), got.astype(np.float32))
     # the refit of its own grid values gives the coefficients back
     t.set_grid_values(t.grid_values())
     assert rel(t.get_coeffs(), cf) <= 1e-12
