@@ -201,10 +201,14 @@ int tbx_semilag_insitu(const tbx_tree *vel, tbx_tfac_fn g, void *gctx,
  *      FP32 coefficient mirrors the trees lack live in a static device
  *      scratch of 256 MiB per GPU (held until the process ends); a step
  *      that needs more returns TBX_EOVERFLOW before any work is queued.
- *      *scratch_bytes (may be NULL) receives the bytes this step needs.
+ *      The velocity's nodal values (TBX_OPT_NODE_VELOCITY) go there too
+ *      when they fit beside those; when they do not, the step evaluates its
+ *      first RK2 stage instead (results then agree with
+ *      tbx_semilag_insitu's to rounding, as with the option off).
+ *      *scratch_bytes (may be NULL) receives the bytes this step uses.
  *      Every call on one GPU shares that GPU's scratch: calls that overlap
- *      on one GPU (two streams, two host threads) race.  Results equal
- *      tbx_semilag_insitu's. ------------------------------------------------ */
+ *      on one GPU (two streams, two host threads) race.  Otherwise results
+ *      equal tbx_semilag_insitu's. ----------------------------------------- */
 int tbx_semilag_insitu_noalloc(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, tbx_tree *conc,
                                int timestep, double dt, int num_rk_step, int bc,
                                uint32_t flags, double *dep, size_t *scratch_bytes,

@@ -221,12 +221,12 @@ def sweep_zalesak(mode="fp64", depths=(4, 5), degrees=(3, 8, 14), revolutions=1)
 
 
 def sweep_zalesak_adaptive(mode="fp64", depths=(5, 7), degrees=(3, 8, 14), revolutions=5,
-                           tol=1e-10):
+                           tol=1e-10, progress=100):
     dt = 6.28e-2
     tn = int(round(6.28 * revolutions / dt))
     for d in depths:
         for q in degrees:
-            yield run_advection_adaptive(2, q, dt, tn, tol, d, mode=mode, progress=100)
+            yield run_advection_adaptive(2, q, dt, tn, tol, d, mode=mode, progress=progress)
 
 
 def main(argv=None) -> int:
@@ -238,6 +238,8 @@ def main(argv=None) -> int:
     ap.add_argument("--depths", default=None, help="comma list (zalesak_adaptive)")
     ap.add_argument("--degrees", default=None, help="comma list (zalesak_adaptive)")
     ap.add_argument("--revolutions", type=float, default=5)
+    ap.add_argument("--progress", type=int, default=100,
+                    help="print the leaf count every this many steps (zalesak_adaptive)")
     ap.add_argument("--mode", default="fp64")
     a = ap.parse_args(argv)
     if a.sweep == "traj":
@@ -250,7 +252,7 @@ def main(argv=None) -> int:
             prev = e
         return 0
     if a.sweep == "zalesak_adaptive":
-        kw = {"revolutions": a.revolutions}
+        kw = {"revolutions": a.revolutions, "progress": a.progress}
         if a.depths:
             kw["depths"] = tuple(int(x) for x in a.depths.split(","))
         if a.degrees:

@@ -448,6 +448,7 @@ int prepare_mode(tbx_tree *t, int mode, hipStream_t s) {
 struct ScratchArena {
   char *base = nullptr;
   size_t size = 0, used = 0;
+  bool nodal = true;  // room for the velocity's nodal values
   void *take(size_t bytes) {
     const size_t off = (used + 255) & ~(size_t)255;
     if (off > size || bytes > size - off) return nullptr;
@@ -1099,7 +1100,7 @@ int step_values(const tbx_velocity *vel, tbx_tree *conc, int timestep, double dt
   }
   new_nodes_host(conc->q, a.node);
   a.vnode = nullptr;
-  if (uses_node_velocity(vel->kind, mode) &&
+  if (uses_node_velocity(vel->kind, mode) && (!arena || arena->nodal) &&
       (rc = node_velocity(const_cast<tbx_tree *>(vel->trees[0]), mode, st, arena, &a.vnode,
                           who)))
     return rc;
@@ -1160,9 +1161,9 @@ int tbx_semilag_insitu_noalloc(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, t
   int mode, rc;
   if ((rc = mode_of(flags, &mode))) return rc;
   // the regions this call needs, each 256-byte aligned: values [L][dof][P],
-  // the refit workspace [L*dof][KP], FP32 mirrors of trees that have none
-  // (FP32 modes) and the velocity's nodal values [L][3][P] (every mode but
-  // TBX_EXACT, with TBX_OPT_NODE_VELOCITY on)
+  // the refit workspace [L*dof][KP] and FP32 mirrors of trees that have
+  // none (FP32 modes); then, if they fit too, the velocity's nodal values
+  // [L][3][P] (every mode but TBX_EXACT, with TBX_OPT_NODE_VELOCITY on)
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t rows = (size_t)conc->L * conc->dof;
   size_t need = up(sizeof(double) * rows * conc->P()) + up(sizeof(double) * rows * conc->KP);
@@ -1170,7 +1171,6 @@ int tbx_semilag_insitu_noalloc(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, t
     if (!vel->d_coeff32) need += up(sizeof(float) * (size_t)vel->ncoeff_total() + kCoeffPad);
     if (!conc->d_coeff32) need += up(sizeof(float) * (size_t)conc->ncoeff_total() + kCoeffPad);
   }
-  if (uses_node_velocity(TBX_VEL_TREE, mode)) need += up(nodal_bytes(vel, mode));
   if (scratch_bytes) *scratch_bytes = need;
   void *base = nullptr;
   size_t cap = 0;
@@ -1181,6 +1181,12 @@ int tbx_semilag_insitu_noalloc(const tbx_tree *vel, tbx_tfac_fn g, void *gctx, t
   ScratchArena arena;
   arena.base = (char *)base;
   arena.size = cap;
+  arena.nodal = false;
+  if (uses_node_velocity(TBX_VEL_TREE, mode) && up(nodal_bytes(vel, mode)) <= cap - need) {
+    need += up(nodal_bytes(vel, mode));
+    arena.nodal = true;
+  }
+  if (scratch_bytes) *scratch_bytes = need;
   double *vals = (double *)arena.take(sizeof(double) * rows * conc->P());
   double *W = (double *)arena.take(sizeof(double) * rows * conc->KP);
   const tbx_velocity v = single_tree(vel, g, gctx);

@@ -1,9 +1,11 @@
 """tbx_semilag_insitu_noalloc (a step that allocates no device memory: its
-value buffer and refit workspace live in the library's static scratch)
-called through the C ABI, against tbx_semilag_insitu on the same inputs:
-bit-identical coefficients and departure points in every arithmetic mode,
-on two calls in a row, the second with a smaller tree, and a step larger
-than the scratch refused before any work is queued."""
+value buffer, refit workspace and the velocity's nodal values live in the
+library's static scratch) called through the C ABI, against
+tbx_semilag_insitu on the same inputs: bit-identical coefficients and
+departure points in every arithmetic mode, on two calls in a row, the
+second with a smaller tree; a step whose nodal values do not fit, which
+evaluates its first stage instead; and a step larger than the scratch
+refused before any work is queued."""
 import ctypes as C
 import math
 
@@ -73,3 +75,34 @@ def test_noalloc_refuses_a_step_beyond_the_scratch(torch, tb):
     assert rc == 6 and need > 256 << 20  # TBX_EOVERFLOW
     assert b"scratch" in tb.lib().tbx_last_error()
     assert torch.equal(conc.get_coeffs_device(), before)
+
+
+def test_noalloc_step_without_room_for_nodal_velocities(torch, tb):
+    # 4,096 leaves at q = 14 in FP64: values and workspace (133 MB) fit the
+    # 256 MiB scratch, the velocity's nodal values (332 MB) do not, so the
+    # step evaluates its first RK2 stage: the same step as the allocating
+    # one (which reads the nodal values) to rounding, and as the allocating
+    # step with the nodal first stage switched off bit for bit.
+    from tbslas_amd import _lib
+
+    g = lambda t: math.cos(math.pi * t)
+    anchor, depth, vcf, ccf = _trees(tb, 4, 14)
+    vel = tb.Tree(anchor, depth, 14, 3, vcf)
+    a = tb.Tree(anchor, depth, 14, 1, ccf)
+    b = tb.Tree(anchor, depth, 14, 1, ccf)
+    c = tb.Tree(anchor, depth, 14, 1, ccf)
+    tb.semilag_insitu(vel, a, 1, 0.01, 1, 1, "fp64", g=g)
+    rc, need = _noalloc(tb, torch, vel, b, 1, 0.01, 1, "fp64", g, None)
+    assert rc == 0, tb.lib().tbx_last_error()
+    nodal = 4096 * 3 * 15 ** 3 * 8
+    assert need <= 256 << 20 and need + nodal > 256 << 20
+    torch.cuda.synchronize()
+    ref = a.get_coeffs()
+    assert np.abs(b.get_coeffs() - ref).max() <= 1e-12 * np.abs(ref).max()
+    old = _lib.get_option(_lib.OPT_NODE_VELOCITY)
+    _lib.set_option(_lib.OPT_NODE_VELOCITY, 0)
+    try:
+        tb.semilag_insitu(vel, c, 1, 0.01, 1, 1, "fp64", g=g)
+    finally:
+        _lib.set_option(_lib.OPT_NODE_VELOCITY, old)
+    assert np.array_equal(c.get_coeffs(), b.get_coeffs())
