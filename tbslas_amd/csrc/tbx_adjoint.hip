@@ -311,6 +311,20 @@ __global__ __launch_bounds__(BLOCK) void k_adj_conc(VelSource vel, TreeView conc
   scatter_transposed<kConcChunk>(lam, dof, bs, weight);
 }
 
+// The first-stage velocity at arrival point i (its wrapped position x, y,
+// z) as the forward step takes it from the velocity's nodal values
+// (a.vnode, FP64; nodal_velocity); false where the forward evaluates it.
+__device__ __forceinline__ bool adj_nodal_stage1(const VelSource &vel, const TreeView &conc,
+                                                 const StepArgs &a, int64_t i, double x,
+                                                 double y, double z, double &v0, double &v1,
+                                                 double &v2) {
+  if (!a.vnode) return false;
+  const int64_t jl = i / a.P;
+  const int lf = locate(vel.t[0], x, y, z);
+  return nodal_velocity<D>((const double *)a.vnode, vel.t[0], conc.info[jl],
+                           (int)(i - jl * a.P), lf, a.P, a.scaled, a.g[0], v0, v1, v2);
+}
+
 // Waves per SIMD asked of the velocity kernel: two fit without spills up
 // to degree 8 (the register budget halves to 256).
 constexpr int kVelWaves = D <= 9 ? 2 : 1;
@@ -354,9 +368,24 @@ __global__ __launch_bounds__(kAdjBlock, kVelWaves) void k_adj_vel(VelSource vel,
     double x, y, z;
     arrival_point(conc, a, i, x, y, z);
 #pragma unroll 1
-    for (int r = 0; r < s; ++r)
-      rk2_substep<D, MODE, kVelTree>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[2 * r],
-                                     a.g[2 * r + 1], a.ts[2 * r], a.ts[2 * r + 1], x, y, z);
+    for (int r = 0; r < s; ++r) {
+      double w0, w1, w2;
+      if (r == 0 && a.vnode) {  // the forward's first stage from the nodal values
+        if (a.bc == TBX_BC_PERIODIC) {
+          x = wrap1(x);
+          y = wrap1(y);
+          z = wrap1(z);
+        }
+        if (!adj_nodal_stage1(vel, conc, a, i, x, y, z, w0, w1, w2))
+          eval_velocity<D, MODE, kVelTree>(vel, 1, a.ts[0], a.scaled, a.g[0], a.bc, x, y, z, w0,
+                                           w1, w2);
+        rk2_finish<D, MODE, kVelTree>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[1], a.ts[1], x, y,
+                                      z, w0, w1, w2);
+      } else {
+        rk2_substep<D, MODE, kVelTree>(vel, a.bc, a.htau, a.tau, a.scaled, a.g[2 * r],
+                                       a.g[2 * r + 1], a.ts[2 * r], a.ts[2 * r + 1], x, y, z);
+      }
+    }
     if (a.bc == TBX_BC_PERIODIC) {
       x = wrap1(x);
       y = wrap1(y);
@@ -365,8 +394,9 @@ __global__ __launch_bounds__(kAdjBlock, kVelWaves) void k_adj_vel(VelSource vel,
     // the forward's midpoint of substep s (rk2_substep, stage 1)
     double v0, v1, v2;
     double xs = x, ys = y, zs = z;
-    eval_velocity<D, MODE, kVelTree>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, xs, ys, zs,
-                                     v0, v1, v2);
+    if (s != 0 || !adj_nodal_stage1(vel, conc, a, i, x, y, z, v0, v1, v2))
+      eval_velocity<D, MODE, kVelTree>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, xs, ys,
+                                       zs, v0, v1, v2);
     double hx = x + a.htau * v0, hy = y + a.htau * v1, hz = z + a.htau * v2;
     if (a.bc == TBX_BC_PERIODIC) {
       hx = wrap1(hx);
@@ -405,6 +435,30 @@ __global__ __launch_bounds__(kAdjBlock, kVelWaves) void k_adj_vel(VelSource vel,
     // work at x: regroup by x's leaf
     int lx = locate(V, ph[6], ph[7], ph[8]);
     double px[12] = {ph[6], ph[7], ph[8], fx * jx, fx * jy, fx * jz, ax, ay, az, jx, jy, jz};
+    if (s == 0 && a.wnode) {
+      // x is the point's arrival node: where that is a grid node of its
+      // velocity leaf, its weights go to the node's slot (the slots are then
+      // projected, G += B^T slots, in one pass per leaf); only the others
+      // scatter, and only workgroups holding one regroup
+      const int64_t gi = i0 + orig;
+      bool rest = false;
+      if (gi < a.npts) {
+        const int64_t jl = gi / a.P;
+        const int mm = node_of(conc.info[jl], V.info[lx], (int)(gi - jl * a.P), D);
+        if (mm >= 0) {
+          double *wn = a.wnode + (int64_t)lx * 3 * a.P + mm;
+          atomicAdd(wn, px[3]);
+          atomicAdd(wn + a.P, px[4]);
+          atomicAdd(wn + 2 * a.P, px[5]);
+          px[3] = 0.0;
+          px[4] = 0.0;
+          px[5] = 0.0;
+        } else {
+          rest = true;
+        }
+      }
+      if (!__syncthreads_or(rest)) continue;  // s == 0: the last substep
+    }
     regroup(sh, orig, lx, px);
     const double wx = px[3], wy = px[4], wz = px[5];
     AdjBasis bx;

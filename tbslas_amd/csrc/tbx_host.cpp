@@ -365,6 +365,7 @@ struct tbx_tree_s {
   RefitTable tab{};
   double *d_adjA = nullptr;  // [L*P][3] per-point velocity cotangent factors
   double *d_nodal = nullptr;  // [L][dof][P] values at the own grid nodes (velocity trees)
+  double *d_adjW = nullptr;   // [L][dof][P] the velocity adjoint's per-node weights
 
   TreeView view() const {
     TreeView v{};
@@ -399,6 +400,7 @@ void free_tree(tbx_tree *t) {
   hipFree(t->d_W);
   hipFree(t->d_adjA);
   hipFree(t->d_nodal);
+  hipFree(t->d_adjW);
   delete t;
 }
 
@@ -1288,7 +1290,23 @@ int adj_vel_step(const AdjCtx &c, tbx_tree *conc, int timestep, double dt, int b
   StepArgs a;
   int rc = adj_args(c, conc, timestep, dt, bc, kAdjBlock, &a);
   if (rc) return rc;
+  // TBX_FP64 with the nodal first stage (as the forward step takes it): the
+  // first substep's arrival nodes add their weights to per-node slots of
+  // the velocity tree, projected into grad_vel afterwards (B^T per leaf
+  // component, k_refit_project) instead of each point scattering
+  tbx_tree *vt = const_cast<tbx_tree *>(c.v.trees[0]);
+  const bool nodal = uses_node_velocity(TBX_VEL_TREE, c.mode);
+  if (nodal) {
+    if ((rc = node_velocity(vt, c.mode, st, nullptr, &a.vnode, "tbx_semilag_insitu_vjp")))
+      return rc;
+    if (!vt->d_adjW) HIP_TRY(hipMalloc(&vt->d_adjW, nodal_bytes(vt)));
+    HIP_TRY(hipMemsetAsync(vt->d_adjW, 0, nodal_bytes(vt), st));
+    a.wnode = vt->d_adjW;
+  }
   HIP_TRY(dispatch_adj_vel(conc->q, c.mode, c.vs, conc->view(), a, A, grad_vel, st));
+  if (nodal)
+    HIP_TRY(launch_refit_project_add(vt->q + 1, vt->d_adjW, vt->L * 3, 3, vt->P(), 1, vt->tab,
+                                     grad_vel, vt->nc, st));
   return TBX_OK;
 }
 
@@ -2435,11 +2453,13 @@ int rebuild(tbx_tree *t, const std::vector<NewLeaf> &leaves, const double *pcoef
   (void)hipFree(t->d_W);
   (void)hipFree(t->d_adjA);
   (void)hipFree(t->d_nodal);
+  (void)hipFree(t->d_adjW);
   t->d_coeff32 = nullptr;
   t->d_vals = nullptr;
   t->d_W = nullptr;
   t->d_adjA = nullptr;
   t->d_nodal = nullptr;
+  t->d_adjW = nullptr;
   t->f32_stale = true;
   return TBX_OK;
 }

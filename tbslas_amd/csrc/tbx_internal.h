@@ -120,6 +120,10 @@ struct StepArgs {
   // FP32 modes and FP64 otherwise (null: the first stage evaluates like the
   // second); single velocity trees only
   const void *vnode;
+  // the velocity adjoint: per velocity grid node [L][3][P], the weights of
+  // the first substep's arrival nodes that are grid nodes of their velocity
+  // leaf (null: every point scatters)
+  double *wnode;
 };
 
 constexpr int kBlock = 256;
@@ -230,6 +234,10 @@ hipError_t launch_refit_gemm(const double *A, int64_t lda, int64_t rows, int K, 
 hipError_t launch_refit_project(int d, const double *V, int64_t rows, int dof, int64_t cs,
                                 int64_t ps, const RefitTable &tab, double *W, int ldw,
                                 hipStream_t s);
+// The same with the result added to W's first Nc entries per row.
+hipError_t launch_refit_project_add(int d, const double *V, int64_t rows, int dof, int64_t cs,
+                                    int64_t ps, const RefitTable &tab, double *W, int ldw,
+                                    hipStream_t s);
 // V = B U per row: U [rows][KP] (packed triangle), V [rows][d^3].
 hipError_t launch_refit_expand(int d, const double *U, int64_t rows, const RefitTable &tab,
                                int KP, double *V, hipStream_t s);

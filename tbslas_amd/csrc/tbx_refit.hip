@@ -155,7 +155,8 @@ constexpr int kRefitThreads = 256;
 // dof + l reads V[leaf P dof + l cs + p ps] at node p = a + D (b + D c)
 // (cs = P, ps = 1 for [L][dof][P] values; cs = 1, ps = dof for point-major
 // [L][P][dof]); W rows have stride ldw, entries Nc .. ldw-1 written as 0.
-template <int D>
+// ACC: w is added to W instead (entries past Nc untouched).
+template <int D, bool ACC>
 __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *__restrict__ V,
                                                                  int64_t rows, int dof,
                                                                  int64_t cs, int64_t ps,
@@ -208,10 +209,14 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
       double s = 0.0;
 #pragma unroll
       for (int c = 0; c < D; ++c) s = __builtin_fma(sT[i * D + c], x[c], s);
-      w[tri_index<D>(i, j, k)] = s;
+      if constexpr (ACC)
+        w[tri_index<D>(i, j, k)] += s;
+      else
+        w[tri_index<D>(i, j, k)] = s;
     }
   }
-  for (int m = NC + tid; m < ldw; m += kRefitThreads) w[m] = 0.0;
+  if constexpr (!ACC)
+    for (int m = NC + tid; m < ldw; m += kRefitThreads) w[m] = 0.0;
   __syncthreads();  // the next row reuses sa and sb
   }
 }
@@ -409,17 +414,17 @@ hipError_t launch_refit_gemm(const double *A, int64_t lda, int64_t rows, int K, 
 // kRowBlocks workgroups, each taking every kRowBlocks-th row.
 constexpr int64_t kRowBlocks = 1 << 20;
 
-hipError_t launch_refit_project(int d, const double *V, int64_t rows, int dof, int64_t cs,
-                                int64_t ps, const RefitTable &tab, double *W, int ldw,
-                                hipStream_t s) {
+template <bool ACC>
+hipError_t launch_project(int d, const double *V, int64_t rows, int dof, int64_t cs, int64_t ps,
+                          const RefitTable &tab, double *W, int ldw, hipStream_t s) {
   if (rows <= 0) return hipSuccess;
   if (dof < 1) return hipErrorInvalidValue;
   const unsigned nb = (unsigned)(rows < kRowBlocks ? rows : kRowBlocks);
   switch (d) {
-#define X(D)                                                                              \
-  case D:                                                                                 \
-    hipLaunchKernelGGL(k_refit_project<D>, dim3(nb), dim3(kRefitThreads), 0, s, V, rows, dof, \
-                       cs, ps, tab, W, ldw);                                              \
+#define X(D)                                                                                 \
+  case D:                                                                                    \
+    hipLaunchKernelGGL((k_refit_project<D, ACC>), dim3(nb), dim3(kRefitThreads), 0, s, V, rows, \
+                       dof, cs, ps, tab, W, ldw);                                            \
     break;
     TBX_REFIT_CASES(X)
 #undef X
@@ -427,6 +432,18 @@ hipError_t launch_refit_project(int d, const double *V, int64_t rows, int dof, i
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_refit_project(int d, const double *V, int64_t rows, int dof, int64_t cs,
+                                int64_t ps, const RefitTable &tab, double *W, int ldw,
+                                hipStream_t s) {
+  return launch_project<false>(d, V, rows, dof, cs, ps, tab, W, ldw, s);
+}
+
+hipError_t launch_refit_project_add(int d, const double *V, int64_t rows, int dof, int64_t cs,
+                                    int64_t ps, const RefitTable &tab, double *W, int ldw,
+                                    hipStream_t s) {
+  return launch_project<true>(d, V, rows, dof, cs, ps, tab, W, ldw, s);
 }
 
 template <typename TO>
