@@ -145,8 +145,16 @@ __device__ __forceinline__ int tri_index(int i, int j, int k) {
 
 constexpr int kRefitThreads = 256;
 
+// Rows a workgroup works on at once: as many teams of D^2 threads (one per
+// (c, b) pair of a row) as the workgroup holds, so that low degrees keep
+// the workgroup busy (3 rows at q = 8, 1 from q = 11).
+template <int D>
+constexpr int refit_rows() {
+  return D * D <= kRefitThreads ? kRefitThreads / (D * D) : 1;
+}
+
 // w = S v per row with S = S_z (x) S_y (x) S_x restricted to the triangle:
-// one workgroup per row (leaf component).
+// one team of threads per row (refit_rows rows per workgroup at a time).
 //   a[c][b][k] = sum_a S[k][a] v[c][b][a]
 //   b[c][j][k] = sum_b S[j][b] a[c][b][k]          j + k < D
 //   w[i][j][k] = sum_c S[i][c] b[c][j][k]          i + j + k < D
@@ -164,60 +172,74 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
                                                                  double *__restrict__ W,
                                                                  int ldw) {
   constexpr int P = D * D * D, NC = D * (D + 1) * (D + 2) / 6;
-  __shared__ double sa[P];
-  __shared__ double sb[P];
+  constexpr int R = refit_rows<D>();
+  constexpr int TS = R > 1 ? D * D : kRefitThreads;  // threads of a row's team
+  __shared__ double sa[R * P];
+  __shared__ double sb[R * P];
   __shared__ double sT[D * D];  // sT[k * D + a] = S[k][a]
   const int tid = threadIdx.x;
+  const int rl = tid / TS < R ? tid / TS : R - 1;  // this thread's row of the group
+  const int tl = tid - (tid / TS) * TS;
+  const bool team = tid < R * TS;
+  double *ra = sa + rl * P, *rb = sb + rl * P;
   for (int t = tid; t < D * D; t += kRefitThreads) sT[t] = tab.T[t / D][t % D];
   __syncthreads();
-  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
-  const double *v = V + (r / dof) * (int64_t)P * dof + (r % dof) * cs;
-  for (int t = tid; t < D * D; t += kRefitThreads) {  // t = c * D + b
-    double x[D];
+  for (int64_t r0 = (int64_t)blockIdx.x * R; r0 < rows; r0 += (int64_t)gridDim.x * R) {
+    const int64_t r = r0 + rl;
+    const bool act = team && r < rows;
+    if (act) {
+      const double *v = V + (r / dof) * (int64_t)P * dof + (r % dof) * cs;
+      for (int t = tl; t < D * D; t += TS) {  // t = c * D + b
+        double x[D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = v[(t * D + a) * ps];
+        for (int a = 0; a < D; ++a) x[a] = v[(t * D + a) * ps];
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      double s = 0.0;
+        for (int k = 0; k < D; ++k) {
+          double s = 0.0;
 #pragma unroll
-      for (int a = 0; a < D; ++a) s = __builtin_fma(sT[k * D + a], x[a], s);
-      sa[t * D + k] = s;
+          for (int a = 0; a < D; ++a) s = __builtin_fma(sT[k * D + a], x[a], s);
+          ra[t * D + k] = s;
+        }
+      }
     }
-  }
-  __syncthreads();
-  for (int t = tid; t < D * D; t += kRefitThreads) {  // t = c * D + k
-    const int c = t / D, k = t % D;
-    double x[D];
+    __syncthreads();
+    if (act) {
+      for (int t = tl; t < D * D; t += TS) {  // t = c * D + k
+        const int c = t / D, k = t % D;
+        double x[D];
 #pragma unroll
-    for (int b = 0; b < D; ++b) x[b] = sa[(c * D + b) * D + k];
-    for (int j = 0; j + k < D; ++j) {
-      double s = 0.0;
+        for (int b = 0; b < D; ++b) x[b] = ra[(c * D + b) * D + k];
+        for (int j = 0; j + k < D; ++j) {
+          double s = 0.0;
 #pragma unroll
-      for (int b = 0; b < D; ++b) s = __builtin_fma(sT[j * D + b], x[b], s);
-      sb[(c * D + j) * D + k] = s;
+          for (int b = 0; b < D; ++b) s = __builtin_fma(sT[j * D + b], x[b], s);
+          rb[(c * D + j) * D + k] = s;
+        }
+      }
     }
-  }
-  __syncthreads();
-  double *w = W + r * ldw;
-  for (int t = tid; t < D * D; t += kRefitThreads) {  // t = j * D + k
-    const int j = t / D, k = t % D;
-    if (j + k >= D) continue;
-    double x[D];
+    __syncthreads();
+    if (act) {
+      double *w = W + r * ldw;
+      for (int t = tl; t < D * D; t += TS) {  // t = j * D + k
+        const int j = t / D, k = t % D;
+        if (j + k >= D) continue;
+        double x[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) x[c] = sb[(c * D + j) * D + k];
-    for (int i = 0; i + j + k < D; ++i) {
-      double s = 0.0;
+        for (int c = 0; c < D; ++c) x[c] = rb[(c * D + j) * D + k];
+        for (int i = 0; i + j + k < D; ++i) {
+          double s = 0.0;
 #pragma unroll
-      for (int c = 0; c < D; ++c) s = __builtin_fma(sT[i * D + c], x[c], s);
-      if constexpr (ACC)
-        w[tri_index<D>(i, j, k)] += s;
-      else
-        w[tri_index<D>(i, j, k)] = s;
+          for (int c = 0; c < D; ++c) s = __builtin_fma(sT[i * D + c], x[c], s);
+          if constexpr (ACC)
+            w[tri_index<D>(i, j, k)] += s;
+          else
+            w[tri_index<D>(i, j, k)] = s;
+        }
+      }
+      if constexpr (!ACC)
+        for (int m = NC + tl; m < ldw; m += TS) w[m] = 0.0;
     }
-  }
-  if constexpr (!ACC)
-    for (int m = NC + tid; m < ldw; m += kRefitThreads) w[m] = 0.0;
-  __syncthreads();  // the next row reuses sa and sb
+    __syncthreads();  // the next rows reuse sa and sb
   }
 }
 
@@ -233,56 +255,70 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__
                                                                 int64_t rows, RefitTable tab,
                                                                 int KP, TO *__restrict__ V) {
   constexpr int P = D * D * D;
-  __shared__ double sa[P];
-  __shared__ double sb[P];
+  constexpr int R = refit_rows<D>();
+  constexpr int TS = R > 1 ? D * D : kRefitThreads;  // threads of a row's team
+  __shared__ double sa[R * P];
+  __shared__ double sb[R * P];
   __shared__ double sT[D * D];
   const int tid = threadIdx.x;
+  const int rl = tid / TS < R ? tid / TS : R - 1;  // this thread's row of the group
+  const int tl = tid - (tid / TS) * TS;
+  const bool team = tid < R * TS;
+  double *ra = sa + rl * P, *rb = sb + rl * P;
   for (int t = tid; t < D * D; t += kRefitThreads) sT[t] = tab.T[t / D][t % D];
   __syncthreads();
-  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
-  const double *u = U + r * KP;
-  for (int t = tid; t < D * D; t += kRefitThreads) {  // t = j * D + k
-    const int j = t / D, k = t % D;
-    if (j + k >= D) continue;
-    double x[D];
-    const int ni = D - j - k;
+  for (int64_t r0 = (int64_t)blockIdx.x * R; r0 < rows; r0 += (int64_t)gridDim.x * R) {
+    const int64_t r = r0 + rl;
+    const bool act = team && r < rows;
+    if (act) {
+      const double *u = U + r * KP;
+      for (int t = tl; t < D * D; t += TS) {  // t = j * D + k
+        const int j = t / D, k = t % D;
+        if (j + k >= D) continue;
+        double x[D];
+        const int ni = D - j - k;
 #pragma unroll
-    for (int i = 0; i < D; ++i) x[i] = i < ni ? u[tri_index<D>(i, j, k)] : 0.0;
-    for (int c = 0; c < D; ++c) {
-      double s = 0.0;
+        for (int i = 0; i < D; ++i) x[i] = i < ni ? u[tri_index<D>(i, j, k)] : 0.0;
+        for (int c = 0; c < D; ++c) {
+          double s = 0.0;
 #pragma unroll
-      for (int i = 0; i < D; ++i) s = __builtin_fma(sT[i * D + c], x[i], s);
-      sb[(c * D + j) * D + k] = s;
+          for (int i = 0; i < D; ++i) s = __builtin_fma(sT[i * D + c], x[i], s);
+          rb[(c * D + j) * D + k] = s;
+        }
+      }
     }
-  }
-  __syncthreads();
-  for (int t = tid; t < D * D; t += kRefitThreads) {  // t = c * D + k
-    const int c = t / D, k = t % D;
-    double x[D];
+    __syncthreads();
+    if (act) {
+      for (int t = tl; t < D * D; t += TS) {  // t = c * D + k
+        const int c = t / D, k = t % D;
+        double x[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) x[j] = j + k < D ? sb[(c * D + j) * D + k] : 0.0;
-    for (int b = 0; b < D; ++b) {
-      double s = 0.0;
+        for (int j = 0; j < D; ++j) x[j] = j + k < D ? rb[(c * D + j) * D + k] : 0.0;
+        for (int b = 0; b < D; ++b) {
+          double s = 0.0;
 #pragma unroll
-      for (int j = 0; j < D; ++j) s = __builtin_fma(sT[j * D + b], x[j], s);
-      sa[(c * D + b) * D + k] = s;
+          for (int j = 0; j < D; ++j) s = __builtin_fma(sT[j * D + b], x[j], s);
+          ra[(c * D + b) * D + k] = s;
+        }
+      }
     }
-  }
-  __syncthreads();
-  TO *v = V + r * P;
-  for (int t = tid; t < D * D; t += kRefitThreads) {  // t = c * D + b
-    double x[D];
+    __syncthreads();
+    if (act) {
+      TO *v = V + r * P;
+      for (int t = tl; t < D * D; t += TS) {  // t = c * D + b
+        double x[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) x[k] = sa[t * D + k];
+        for (int k = 0; k < D; ++k) x[k] = ra[t * D + k];
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-      double s = 0.0;
+        for (int a = 0; a < D; ++a) {
+          double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < D; ++k) s = __builtin_fma(sT[k * D + a], x[k], s);
-      v[t * D + a] = (TO)s;
+          for (int k = 0; k < D; ++k) s = __builtin_fma(sT[k * D + a], x[k], s);
+          v[t * D + a] = (TO)s;
+        }
+      }
     }
-  }
-  __syncthreads();  // the next row reuses sa and sb
+    __syncthreads();  // the next rows reuse sa and sb
   }
 }
 
@@ -419,13 +455,15 @@ hipError_t launch_project(int d, const double *V, int64_t rows, int dof, int64_t
                           const RefitTable &tab, double *W, int ldw, hipStream_t s) {
   if (rows <= 0) return hipSuccess;
   if (dof < 1) return hipErrorInvalidValue;
-  const unsigned nb = (unsigned)(rows < kRowBlocks ? rows : kRowBlocks);
   switch (d) {
 #define X(D)                                                                                 \
-  case D:                                                                                    \
+  case D: {                                                                                  \
+    const int64_t groups = (rows + refit_rows<D>() - 1) / refit_rows<D>();                   \
+    const unsigned nb = (unsigned)(groups < kRowBlocks ? groups : kRowBlocks);               \
     hipLaunchKernelGGL((k_refit_project<D, ACC>), dim3(nb), dim3(kRefitThreads), 0, s, V, rows, \
                        dof, cs, ps, tab, W, ldw);                                            \
-    break;
+    break;                                                                                   \
+  }
     TBX_REFIT_CASES(X)
 #undef X
     default:
@@ -453,13 +491,15 @@ hipError_t launch_expand(int d, const double *U, int64_t rows, const RefitTable 
   // 8,192 row-strided workgroups: 9 % faster than one per row on 108,972
   // rows of degree 14 (the table set-up is paid once per workgroup)
   constexpr int64_t kExpandBlocks = 8192;
-  const unsigned nb = (unsigned)(rows < kExpandBlocks ? rows : kExpandBlocks);
   switch (d) {
 #define X(D)                                                                             \
-  case D:                                                                                \
+  case D: {                                                                              \
+    const int64_t groups = (rows + refit_rows<D>() - 1) / refit_rows<D>();               \
+    const unsigned nb = (unsigned)(groups < kExpandBlocks ? groups : kExpandBlocks);     \
     hipLaunchKernelGGL((k_refit_expand<D, TO>), dim3(nb), dim3(kRefitThreads), 0, s, U, rows, \
                        tab, KP, V);                                                      \
-    break;
+    break;                                                                               \
+  }
     TBX_REFIT_CASES(X)
 #undef X
     default:
