@@ -193,12 +193,21 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
         double x[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) x[a] = v[(t * D + a) * ps];
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
+        auto out = [&](int k) {
           double s = 0.0;
 #pragma unroll
           for (int a = 0; a < D; ++a) s = __builtin_fma(sT[k * D + a], x[a], s);
           ra[t * D + k] = s;
+        };
+        // several rows at once: one output at a time keeps the registers
+        // (and the waves per SIMD) that the rows' latency needs; one row:
+        // all outputs unrolled for their independent chains
+        if constexpr (R > 1) {
+#pragma unroll 1
+          for (int k = 0; k < D; ++k) out(k);
+        } else {
+#pragma unroll
+          for (int k = 0; k < D; ++k) out(k);
         }
       }
     }
@@ -309,12 +318,18 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__
         double x[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) x[k] = ra[t * D + k];
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
+        auto out = [&](int a) {
           double s = 0.0;
 #pragma unroll
           for (int k = 0; k < D; ++k) s = __builtin_fma(sT[k * D + a], x[k], s);
           v[t * D + a] = (TO)s;
+        };
+        if constexpr (R > 1) {  // as in k_refit_project
+#pragma unroll 1
+          for (int a = 0; a < D; ++a) out(a);
+        } else {
+#pragma unroll
+          for (int a = 0; a < D; ++a) out(a);
         }
       }
     }
