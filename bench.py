@@ -704,6 +704,8 @@ def main():
                     help="step kernel launch shape (default -1: per arithmetic mode)")
     ap.add_argument("--node-velocity", type=int, choices=(0, 1), default=None,
                     help="TBX_OPT_NODE_VELOCITY for every step (default: the library's, 1)")
+    ap.add_argument("--grid-lds", type=int, choices=(0, 1, 2), default=None,
+                    help="TBX_OPT_GRID_LDS for the tricubic steps (default: the library's, 1)")
     ap.add_argument("--profile", default=None, metavar="MODE",
                     help="counter-collection run: a 1 GiB device copy (the reference "
                          "for byte counters), then --steps step kernels of one "
@@ -746,6 +748,8 @@ def main():
 
     if args.node_velocity is not None:
         _lib.set_option(_lib.OPT_NODE_VELOCITY, args.node_velocity)
+    if args.grid_lds is not None:
+        _lib.set_option(_lib.OPT_GRID_LDS, args.grid_lds)
 
     if args.batch_only:
         print(json.dumps({"batch_256_problems": batch_measure(2, 256)}))

@@ -615,6 +615,9 @@ __device__ __forceinline__ int node_of(uint4 own, uint4 o, int m, int D) {
     const int ak = rest % D;
     rest /= D;
     const uint32_t d = (ov[k] - nv[k]) & mask;
+    // a single leaf (depth 0) is its own neighbour across the periodic
+    // wrap, so an upper-face node may stand for node 0: left to evaluation
+    if (mask == 0 && ak == D - 1) return -1;
     int bk;
     if (d == 0)
       bk = ak;

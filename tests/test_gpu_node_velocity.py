@@ -204,3 +204,22 @@ def test_grid_values_against_oracle(torch, tb, orc, q, dof):
     # the refit of its own grid values gives the coefficients back
     t.set_grid_values(t.grid_values())
     assert rel(t.get_coeffs(), cf) <= 1e-12
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_single_leaf_tree(torch, tb, bc):
+    # depth 0: the leaf is its own periodic neighbour, so its upper-face
+    # nodes (wrapped to 0) must not read the node at the upper face; a
+    # velocity that differs across the periodic boundary shows it
+    q = 8
+    anchor, depth = tb.uniform_leaves(0)
+    vcf = tb.fit_function(lambda p: drift(p) + 0.1 * p, anchor, depth, q, 3)
+    ccf = tb.fit_function(lambda p: np.exp(-((p - 0.4) ** 2).sum(1, keepdims=True) / 0.05),
+                          anchor, depth, q, 1)
+    vel = tb.Tree(anchor, depth, q, 3, vcf)
+    conc = tb.Tree(anchor, depth, q, 1, ccf)
+    v1, d1 = _run(torch, tb, vel, conc, "fp64", bc, 1, True)
+    v0, d0 = _run(torch, tb, vel, conc, "fp64", bc, 1, False)
+    dd = periodic_diff(d1, d0) if bc else np.abs(d1 - d0).max()
+    assert dd <= 1e-15, dd
+    assert rel(v1, v0) <= 1e-13
