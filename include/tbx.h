@@ -457,7 +457,8 @@ int tbx_balance21(int64_t *num_leaves, int64_t capacity, uint32_t *anchor,
  * leaves whose velocity and concentration trees share their leaves stages each
  * sibling block of 8 leaves in LDS, in FP32 steps (1) or in every mode (2); 0
  * gathers every stencil from global memory.  All give the same bits.
- * TBX_OPT_NODE_VELOCITY (default 1): a step with a single velocity tree, in
+ * TBX_OPT_NODE_VELOCITY (default 1): a step with a single velocity tree (or
+ * the extrapolating functor, whose first stage is its current tree), in
  * any mode but TBX_EXACT, expands the velocity tree at its own grid nodes
  * (three one-axis contractions per leaf, into a buffer of L * 3 * (q+1)^3
  * doubles the tree keeps) and takes the first RK2 stage's velocity at each

@@ -1018,10 +1018,13 @@ int tbx_tree_grid_values_f32(const tbx_tree *t, float *vals, tbx_stream s) {
 namespace {
 
 // Whether a step's first stage reads the velocity tree's nodal values
-// (TBX_OPT_NODE_VELOCITY): single velocity trees, not in TBX_EXACT (which
-// evaluates every stage as the reference does, bit for bit).
+// (TBX_OPT_NODE_VELOCITY): a single velocity tree, or the extrapolating
+// functor (whose first stage is its current tree alone, traj.inc:80); not
+// in TBX_EXACT (which evaluates every stage as the reference does, bit for
+// bit).
 bool uses_node_velocity(int vel_kind, int mode) {
-  return g_node_velocity.load() && vel_kind == TBX_VEL_TREE && mode != kExact64;
+  return g_node_velocity.load() && (vel_kind == TBX_VEL_TREE || vel_kind == TBX_VEL_EXTRAP) &&
+         mode != kExact64;
 }
 
 // Bytes of a tree's nodal values in a mode (FP32 in the FP32 modes); the

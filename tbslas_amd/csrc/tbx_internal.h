@@ -118,7 +118,8 @@ struct StepArgs {
   double node[TBX_MAX_Q + 1];  // stretched nodes (cheb.h:53-58)
   // the velocity tree's values at its own grid nodes, [L][3][P], FP32 in the
   // FP32 modes and FP64 otherwise (null: the first stage evaluates like the
-  // second); single velocity trees only
+  // second); single velocity trees and the extrapolating functor's current
+  // tree (its first stage) only
   const void *vnode;
   // the velocity adjoint: per velocity grid node [L][3][P], the weights of
   // the first substep's arrival nodes that are grid nodes of their velocity

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
   double z = (double)inf.z * len + len * a.node[m / (D * D)];
   // ComputeTrajRK2 (traj.inc:48-68), tinit -> tfinal = tinit - dt
   int s = 0;
-  if constexpr (KIND == kVelTree) {
+  if constexpr (KIND == kVelTree || KIND == kVelExtrap) {
     if (a.vnode) {  // the first stage at the arrival node from the nodal values
       if (a.bc == TBX_BC_PERIODIC) {
         x = wrap1(x);
@@ -221,7 +221,7 @@ __device__ __forceinline__ void k_step_rg_body(const VelSource &vel, const TreeV
       lf = locate(vel.t[0], st[0], st[1], st[2]);
       double v0, v1, v2;
       bool nodal = false;
-      if constexpr (KIND == kVelTree) {
+      if constexpr (KIND == kVelTree || KIND == kVelExtrap) {
         // the first substep starts at the arrival nodes: their velocities
         // from the nodal values, and the regroup and contraction only in
         // workgroups holding a node of another depth's leaf (whose points
@@ -249,7 +249,7 @@ __device__ __forceinline__ void k_step_rg_body(const VelSource &vel, const TreeV
         eval_velocity<D, MODE, KIND>(vel, 1, a.ts[2 * s], a.scaled, a.g[2 * s], a.bc, st[0],
                                      st[1], st[2], v0, v1, v2, lf);
         reload_start(sh, st);
-        if constexpr (KIND == kVelTree) {
+        if constexpr (KIND == kVelTree || KIND == kVelExtrap) {
           if (s == 0 && a.vnode) {  // the point this thread now holds
             int64_t jl;
             int jm;

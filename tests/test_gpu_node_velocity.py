@@ -223,3 +223,32 @@ def test_single_leaf_tree(torch, tb, bc):
     dd = periodic_diff(d1, d0) if bc else np.abs(d1 - d0).max()
     assert dd <= 1e-15, dd
     assert rel(v1, v0) <= 1e-13
+
+
+def test_nodal_first_stage_extrapolated_velocity(torch, tb):
+    # the extrapolating functor's first stage is its current tree alone
+    # (traj.inc:80): it reads that tree's nodal values; the second stage
+    # extrapolates as before
+    q = 8
+    (va, vd, vcf), (anchor, depth, ccf) = _setup(tb, q, "same", seed=16)
+    cur = tb.Tree(va, vd, q, 3, vcf)
+    prev = tb.Tree(va, vd, q, 3, 0.9 * vcf)
+    conc = tb.Tree(anchor, depth, q, 1, ccf)
+    vel = tb.Velocity.extrap(cur, prev)
+    from tbslas_amd import _lib
+
+    out = {}
+    for on in (True, False):
+        old = _lib.get_option(_lib.OPT_NODE_VELOCITY)
+        _lib.set_option(_lib.OPT_NODE_VELOCITY, 1 if on else 0)
+        try:
+            for mode in ("fp64", "fp32"):
+                dep = torch.empty((conc.L * conc.P, 3), dtype=torch.float64, device="cuda")
+                v = tb.semilag_values(vel, conc, 2, 0.02, 1, 1, mode, dep=dep)
+                out[(on, mode)] = (v.cpu().numpy(), dep.cpu().numpy())
+        finally:
+            _lib.set_option(_lib.OPT_NODE_VELOCITY, old)
+    for mode, dtol, vtol in (("fp64", 1e-15, 1e-13), ("fp32", 1e-8, 1e-6)):
+        (v1, d1), (v0, d0) = out[(True, mode)], out[(False, mode)]
+        assert periodic_diff(d1, d0) <= dtol, mode
+        assert rel(v1, v0) <= vtol, mode

@@ -293,7 +293,8 @@ def test_velocity_functors_against_oracle(torch, tb, orc, kind):
         assert np.abs(got - ref).max() <= 1e-13
 
 
-@pytest.mark.parametrize("kind,mode", [("extrap", "exact"), ("set", "exact"), ("set", "fp64")])
+@pytest.mark.parametrize("kind,mode", [("extrap", "exact"), ("extrap", "fp64"), ("set", "exact"),
+                                       ("set", "fp64")])
 def test_insitu_velocity_functors_against_oracle(torch, tb, orc, kind, mode):
     from tbslas_amd import fields
 
