@@ -153,6 +153,13 @@ constexpr int refit_rows() {
   return D * D <= kRefitThreads ? kRefitThreads / (D * D) : 1;
 }
 
+// Whether a thread loads its first line of the next row ahead (when each
+// thread has one line per stage; above q = 15 the registers are spent).
+template <int D>
+constexpr bool refit_prefetch() {
+  return D * D <= kRefitThreads;
+}
+
 // w = S v per row with S = S_z (x) S_y (x) S_x restricted to the triangle:
 // one team of threads per row (refit_rows rows per workgroup at a time).
 //   a[c][b][k] = sum_a S[k][a] v[c][b][a]
@@ -183,16 +190,34 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
   const bool team = tid < R * TS;
   double *ra = sa + rl * P, *rb = sb + rl * P;
   for (int t = tid; t < D * D; t += kRefitThreads) sT[t] = tab.T[t / D][t % D];
+  // The first (c, b) line of the thread's next row is loaded while the
+  // current row goes through its LDS stages, so that each row does not
+  // wait for its own loads (two workgroups per CU at q = 14).
+  const int64_t stride = (int64_t)gridDim.x * R;
+  double xn[D];
+  auto fetch = [&](int64_t r) {
+    if (refit_prefetch<D>() && team && tl < D * D && r < rows) {
+      const double *v = V + (r / dof) * (int64_t)P * dof + (r % dof) * cs;
+#pragma unroll
+      for (int a = 0; a < D; ++a) xn[a] = v[(tl * D + a) * ps];
+    }
+  };
+  fetch((int64_t)blockIdx.x * R + rl);
   __syncthreads();
-  for (int64_t r0 = (int64_t)blockIdx.x * R; r0 < rows; r0 += (int64_t)gridDim.x * R) {
+  for (int64_t r0 = (int64_t)blockIdx.x * R; r0 < rows; r0 += stride) {
     const int64_t r = r0 + rl;
     const bool act = team && r < rows;
     if (act) {
       const double *v = V + (r / dof) * (int64_t)P * dof + (r % dof) * cs;
       for (int t = tl; t < D * D; t += TS) {  // t = c * D + b
         double x[D];
+        if (refit_prefetch<D>() && t == tl) {
 #pragma unroll
-        for (int a = 0; a < D; ++a) x[a] = v[(t * D + a) * ps];
+          for (int a = 0; a < D; ++a) x[a] = xn[a];
+        } else {
+#pragma unroll
+          for (int a = 0; a < D; ++a) x[a] = v[(t * D + a) * ps];
+        }
         auto out = [&](int k) {
           double s = 0.0;
 #pragma unroll
@@ -211,6 +236,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
         }
       }
     }
+    fetch(r + stride);
     __syncthreads();
     if (act) {
       for (int t = tl; t < D * D; t += TS) {  // t = c * D + k
@@ -275,8 +301,22 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__
   const bool team = tid < R * TS;
   double *ra = sa + rl * P, *rb = sb + rl * P;
   for (int t = tid; t < D * D; t += kRefitThreads) sT[t] = tab.T[t / D][t % D];
+  // the first (j, k) line of the thread's next row loaded during the
+  // current row's LDS stages, as in k_refit_project
+  const int64_t stride = (int64_t)gridDim.x * R;
+  double xn[D];
+  auto fetch = [&](int64_t r) {
+    const int j = tl / D, k = tl % D;
+    if (refit_prefetch<D>() && team && tl < D * D && j + k < D && r < rows) {
+      const double *u = U + r * KP;
+      const int ni = D - j - k;
+#pragma unroll
+      for (int i = 0; i < D; ++i) xn[i] = i < ni ? u[tri_index<D>(i, j, k)] : 0.0;
+    }
+  };
+  fetch((int64_t)blockIdx.x * R + rl);
   __syncthreads();
-  for (int64_t r0 = (int64_t)blockIdx.x * R; r0 < rows; r0 += (int64_t)gridDim.x * R) {
+  for (int64_t r0 = (int64_t)blockIdx.x * R; r0 < rows; r0 += stride) {
     const int64_t r = r0 + rl;
     const bool act = team && r < rows;
     if (act) {
@@ -286,8 +326,13 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__
         if (j + k >= D) continue;
         double x[D];
         const int ni = D - j - k;
+        if (refit_prefetch<D>() && t == tl) {
 #pragma unroll
-        for (int i = 0; i < D; ++i) x[i] = i < ni ? u[tri_index<D>(i, j, k)] : 0.0;
+          for (int i = 0; i < D; ++i) x[i] = xn[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < D; ++i) x[i] = i < ni ? u[tri_index<D>(i, j, k)] : 0.0;
+        }
         for (int c = 0; c < D; ++c) {
           double s = 0.0;
 #pragma unroll
@@ -296,6 +341,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_expand(const double *__
         }
       }
     }
+    fetch(r + stride);
     __syncthreads();
     if (act) {
       for (int t = tl; t < D * D; t += TS) {  // t = c * D + k
