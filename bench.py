@@ -604,7 +604,8 @@ def batch_measure(steps: int = 2, nprob: int = 256):
            "tune_ms": {f"{t}x{lpi}": float(m) for (t, lpi), m in zip(CONFIGS, ms)},
            "ms_per_step": step_ms, "interps_per_s": 3 * npts / (step_ms * 1e-3),
            "departure_points_per_s": npts / (step_ms * 1e-3), "dtype": "f32",
-           "coeff_linf_rel_vs_fp64_steps": worst, "steps_compared": taken}
+           "coeff_linf_rel_vs_fp64_steps": worst, "steps_compared": taken,
+           "first_stage": "nodal values" if b.node_velocity else "evaluated"}
     b.close()
     del vel, conc, init
     torch.cuda.empty_cache()

@@ -338,14 +338,21 @@ int tbx_tree_write_vtu(const tbx_tree *t, const char *prefix, int order, tbx_str
  * batch keeps its own FP32 coefficients (initialised from conc[p]); the
  * trees provide the leaf structure and must outlive the batch.  The
  * configuration (workgroup size x leaves per item) is chosen by
- * tbx_batch_tune, which times every one on the current state.  No
- * counterpart in the reference, whose ranks each own one tree. */
+ * tbx_batch_tune, which times every one on the current state.  With
+ * TBX_OPT_NODE_VELOCITY on when the batch is made, the velocities are
+ * expanded at their grid nodes once (FP32, 12 bytes per velocity grid
+ * point, kept only if they take at most half of the free device memory)
+ * and each problem's first RK2 stage reads them where an arrival node is a
+ * node of its velocity leaf, as tbx_semilag_insitu does.  No counterpart
+ * in the reference, whose ranks each own one tree. */
 typedef struct tbx_batch_s tbx_batch;
 int tbx_batch_create(int nprob, const tbx_tree *const *vel, const tbx_tree *const *conc,
                      const double *dt, int bc, tbx_batch **out);
 int tbx_batch_destroy(tbx_batch *b);
 int tbx_batch_set_config(tbx_batch *b, int cfg);       /* 0..5 */
 int tbx_batch_get_config(const tbx_batch *b, int *cfg);
+/* *on = 1 if the batch's first RK2 stage reads the velocities' nodal values. */
+int tbx_batch_node_velocity(const tbx_batch *b, int *on);
 /* Times each configuration (ms[6], may be NULL), keeps the fastest. */
 int tbx_batch_tune(tbx_batch *b, double *ms, tbx_stream s);
 int tbx_batch_step(tbx_batch *b, int nsteps, tbx_stream s);

@@ -127,6 +127,7 @@ _SIGS = {
     "tbx_batch_destroy": (_I32, [_P]),
     "tbx_batch_set_config": (_I32, [_P, _I32]),
     "tbx_batch_get_config": (_I32, [_P, C.POINTER(_I32)]),
+    "tbx_batch_node_velocity": (_I32, [_P, C.POINTER(_I32)]),
     "tbx_batch_tune": (_I32, [_P, _P, _P]),
     "tbx_batch_step": (_I32, [_P, _I32, _P]),
     "tbx_batch_attach_step": (_I32, [_P, _P, _P, _D, _I32, _U32]),

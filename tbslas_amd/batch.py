@@ -46,6 +46,14 @@ class Batch:
         check(lib().tbx_batch_get_config(self._h, C.byref(c)))
         return c.value
 
+    @property
+    def node_velocity(self) -> bool:
+        """Whether the first RK2 stage reads the velocities' nodal values
+        (decided when the batch was made)."""
+        c = C.c_int()
+        check(lib().tbx_batch_node_velocity(self._h, C.byref(c)))
+        return bool(c.value)
+
     def set_config(self, cfg: int) -> None:
         check(lib().tbx_batch_set_config(self._h, cfg))
 

@@ -12,7 +12,10 @@
 // (c'[m] = sum_p M[p][m] v[p], the per-leaf product of SetTreeGridValues,
 // tree_utils.h:534-546, summed in FP64), so no value buffer exists.  Coefficients
 // are FP32 and ping-pong between two buffers; the step of parity k reads
-// buffer k and writes buffer 1 - k.
+// buffer k and writes buffer 1 - k.  With vnode (the stationary velocity's
+// values at its own grid nodes, expanded once when the batch is made) the
+// first RK2 stage at the arrival nodes reads them, as the single-tree step
+// does.
 #include <hip/hip_runtime.h>
 
 #include "tbx_device.h"
@@ -61,16 +64,32 @@ __device__ __forceinline__ void batch_item(const BatchProblem &pb, int parity, i
       st[2] = wrap1(st[2]);
     }
     int lf = locate(pb.vel, st[0], st[1], st[2]);
-    {
-      double p3[3] = {st[0], st[1], st[2]};
-      regroup(sh, orig, lf, p3);
-      st[0] = p3[0];
-      st[1] = p3[1];
-      st[2] = p3[2];
-    }
     double v0, v1, v2;
-    eval_velocity<D, kFp32, kVelTree>(vs, 1, 0.0, 0, 1.0, pb.bc, st[0], st[1], st[2], v0, v1,
-                                      v2, lf);
+    // stage 1 from the nodal values; the regroup and contraction only in
+    // workgroups holding a node of another depth's leaf, whose points then
+    // take the nodal value again where they have one (a point's result does
+    // not depend on the workgroup it falls in)
+    bool nodal = false;
+    if (pb.vnode) {
+      const bool got = nodal_velocity<D>(pb.vnode, pb.vel, inf, m, lf, P, 0, 1.0, v0, v1, v2);
+      nodal = !__syncthreads_or(!got);
+    }
+    if (!nodal) {
+      {
+        double p3[3] = {st[0], st[1], st[2]};
+        regroup(sh, orig, lf, p3);
+        st[0] = p3[0];
+        st[1] = p3[1];
+        st[2] = p3[2];
+      }
+      eval_velocity<D, kFp32, kVelTree>(vs, 1, 0.0, 0, 1.0, pb.bc, st[0], st[1], st[2], v0, v1,
+                                        v2, lf);
+      if (pb.vnode) {  // the arrival node this thread now holds
+        const int jdx = start + orig < total ? start + orig : total - 1;
+        const int64_t jl = leaf0 + jdx / P;
+        nodal_velocity<D>(pb.vnode, pb.vel, conc.info[jl], jdx % P, lf, P, 0, 1.0, v0, v1, v2);
+      }
+    }
     st[3] = st[0] + pb.htau * v0;  // traj.inc:36
     st[4] = st[1] + pb.htau * v1;
     st[5] = st[2] + pb.htau * v2;

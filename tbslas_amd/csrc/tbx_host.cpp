@@ -1668,6 +1668,7 @@ struct tbx_batch_s {
   std::vector<double> dt;
   std::vector<BatchProblem> host;  // table for the current configuration
   float *d_vel32 = nullptr, *d_conc32[2] = {nullptr, nullptr};
+  float *d_vnode = nullptr;        // the velocities at their grid nodes, or null
   std::map<int, double *> d_M;     // refit operator per degree
   BatchProblem *d_table = nullptr;
   unsigned long long *d_counter = nullptr;
@@ -1704,6 +1705,7 @@ void free_batch(tbx_batch *b) {
   free_graphs(b);
   if (b->cap) hipStreamDestroy(b->cap);
   hipFree(b->d_vel32);
+  hipFree(b->d_vnode);
   hipFree(b->d_conc32[0]);
   hipFree(b->d_conc32[1]);
   for (auto &kv : b->d_M) hipFree(kv.second);
@@ -1821,7 +1823,17 @@ int tbx_batch_create(int nprob, const tbx_tree *const *vel, const tbx_tree *cons
       HIP_TRY(hipMemcpy(d, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice));
     }
     b->host.resize(nprob);
-    int64_t ov = 0;
+    // the stationary velocities at their grid nodes for the first RK2
+    // stage (TBX_OPT_NODE_VELOCITY when the batch is made), if they take at
+    // most half of the free device memory; evaluated otherwise
+    size_t nodal = 0;
+    for (int p = 0; p < nprob; ++p) nodal += nodal_bytes(vel[p], kFp32);
+    if (uses_node_velocity(TBX_VEL_TREE, kFp32)) {
+      size_t freeb = 0, totalb = 0;
+      HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+      if (nodal <= freeb / 2) HIP_TRY(hipMalloc(&b->d_vnode, nodal));
+    }
+    int64_t ov = 0, onode = 0;
     for (int p = 0; p < nprob; ++p) {
       const tbx_tree *tv = vel[p], *tc = conc[p];
       HIP_TRY(launch_to_f32(tv->d_coeff, b->d_vel32 + ov, tv->ncoeff_total(), nullptr));
@@ -1838,6 +1850,14 @@ int tbx_batch_create(int nprob, const tbx_tree *const *vel, const tbx_tree *cons
       pb.out32[0] = b->d_conc32[1] + b->conc_off[p];
       pb.out32[1] = b->d_conc32[0] + b->conc_off[p];
       pb.M = b->d_M[tc->q];
+      if (b->d_vnode) {
+        float *vn = b->d_vnode + onode;
+        const int erc = expand_grid_values(const_cast<tbx_tree *>(tv), nullptr, vn, nullptr,
+                                           "tbx_batch_create");
+        if (erc) return erc;
+        pb.vnode = vn;
+        onode += (int64_t)tv->L * 3 * tv->P();
+      }
       pb.L = tc->L;
       pb.q = tc->q;
       pb.bc = bc;
@@ -1875,6 +1895,12 @@ int tbx_batch_set_config(tbx_batch *b, int cfg) {
 int tbx_batch_get_config(const tbx_batch *b, int *cfg) {
   if (!b || !cfg) return fail(TBX_EINVAL, "tbx_batch_get_config: null argument");
   *cfg = b->cfg;
+  return TBX_OK;
+}
+
+int tbx_batch_node_velocity(const tbx_batch *b, int *on) {
+  if (!b || !on) return fail(TBX_EINVAL, "tbx_batch_node_velocity: null argument");
+  *on = b->d_vnode ? 1 : 0;
   return TBX_OK;
 }
 

@@ -193,6 +193,7 @@ struct BatchProblem {
   TreeView conc[2];     // coeff32: the FP32 state read by a step of parity k
   float *out32[2];      // written by a step of parity k (= conc[1 - k].coeff32)
   const double *M;      // [P][Nc] refit operator
+  const float *vnode;   // the velocity at its grid nodes [Lv][3][P] (FP32), or null
   int64_t item0;        // first work item
   int64_t L;            // concentration leaves
   int32_t q;

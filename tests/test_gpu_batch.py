@@ -81,7 +81,8 @@ def _small_problems(tb, n=12, seed=33):
     return probs
 
 
-def test_batch_matches_per_problem_steps(torch, tb):
+@pytest.mark.parametrize("nodal", [1, 0])
+def test_batch_matches_per_problem_steps(torch, tb, nodal):
     from tbslas_amd.batch import CONFIGS, Batch
 
     probs = _small_problems(tb)
@@ -93,40 +94,101 @@ def test_batch_matches_per_problem_steps(torch, tb):
     # every step the same map).  Against FP64 the difference would be the
     # face flips of FP32 departure points on these coarse problems (leaf
     # faces the rotation axis crosses, fit jumps ~1e-4), not rounding.
-    # The batch evaluates every RK2 stage, so the references do too
-    # (TBX_OPT_NODE_VELOCITY off: no first stage from nodal values).
+    # TBX_OPT_NODE_VELOCITY is the same for both: on (the default), the
+    # batch, made while it is on, and the references take the first RK2
+    # stage from the velocities' FP32 nodal values; off, both evaluate it.
     from tbslas_amd import _lib
 
     refs = []
+    results = []
     old = _lib.get_option(_lib.OPT_NODE_VELOCITY)
-    _lib.set_option(_lib.OPT_NODE_VELOCITY, 0)
+    _lib.set_option(_lib.OPT_NODE_VELOCITY, nodal)
     try:
         for (a, d, q, v, c, dt), tv in zip(probs, vel):
             t = tb.Tree(a, d, q, 1, c)
             for _ in range(3):
                 tb.semilag_insitu(tv, t, 1, dt, 1, 0, "fp32")
             refs.append(t.get_coeffs())
+        for cfg in range(len(CONFIGS)):
+            b = Batch(vel, conc, dts, 0)
+            b.set_config(cfg)
+            b.step(3)  # graph replays
+            results.append([b.coeffs(p) for p in range(len(probs))])
+            b.close()
+        b = Batch(vel, conc, dts, 0)
     finally:
         _lib.set_option(_lib.OPT_NODE_VELOCITY, old)
-    results = []
-    for cfg in range(len(CONFIGS)):
-        b = Batch(vel, conc, dts, 0)
-        b.set_config(cfg)
-        b.step(3)  # graph replays
-        results.append([b.coeffs(p) for p in range(len(probs))])
-        b.close()
     for cfg in range(1, len(CONFIGS)):  # configurations change the work split only
         for p in range(len(probs)):
             assert np.array_equal(results[cfg][p], results[0][p]), (cfg, p)
     for p in range(len(probs)):  # the batch keeps its state in FP32
         err = np.abs(results[0][p] - refs[p]).max() / np.abs(refs[p]).max()
         assert err <= 1e-6, (p, err)
-    b = Batch(vel, conc, dts, 0)
     ms = b.tune()
     assert (ms > 0).all() and b.config == int(np.argmin(ms))
     b.step(3)
     for p in range(len(probs)):
         assert np.array_equal(b.coeffs(p), results[0][p])
+
+
+def _drift(p):
+    # total degree 2, exact at q >= 2: continuous across leaf faces and the
+    # periodic boundary, no vanishing component (FP32 rounding of the first
+    # stage moves departure points by rounding only)
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    return np.stack([0.3 + 0.1 * y * (1 - y), -0.2 + 0.1 * z * (1 - z),
+                     0.25 + 0.1 * x * (1 - x)], axis=1)
+
+
+def test_batch_nodal_first_stage_on_mixed_depths(torch, tb):
+    # The batch's first RK2 stage from the velocities' nodal values where the
+    # velocity leaf holding an arrival node has the arrival leaf's depth, and
+    # evaluated elsewhere (nodes on faces towards a leaf of another depth, and
+    # every node under a velocity of another depth).  Problems: velocity and
+    # concentration on one mixed depth-2/3 tree, and a uniform depth-2
+    # velocity under a mixed concentration; degrees 6 and 8; periodic.
+    # Against per-problem FP32 steps that take the same first stage (each
+    # point's result is the same in both) at 1e-6, and against the batch
+    # made with TBX_OPT_NODE_VELOCITY off (the first stage evaluated in FP32:
+    # rounding differences only) at 2e-6.
+    from conftest import random_tree
+    from tbslas_amd import _lib, fields
+    from tbslas_amd.batch import Batch
+
+    probs = []
+    for p, (q, coarse) in enumerate(((6, False), (8, False), (6, True), (8, True))):
+        anchor, depth = random_tree(50 + p, max_depth=3, min_depth=2)
+        va, vd = tb.uniform_leaves(2) if coarse else (anchor, depth)
+        vcf = tb.fit_function(_drift, va, vd, q, 3)
+        ccf = tb.fit_function(lambda x: fields.gaussian(x, (0.55, 0.5, 0.45), 0.12), anchor,
+                              depth, q, 1)
+        probs.append(((va, vd, vcf), (anchor, depth, ccf), q, 0.004 * (1 + p % 2)))
+    vel = [tb.Tree(v[0], v[1], q, 3, v[2]) for v, c, q, dt in probs]
+    dts = [dt for *_, dt in probs]
+    got = {}
+    refs = []
+    old = _lib.get_option(_lib.OPT_NODE_VELOCITY)
+    try:
+        for on in (1, 0):
+            _lib.set_option(_lib.OPT_NODE_VELOCITY, on)
+            conc = [tb.Tree(c[0], c[1], q, 1, c[2]) for v, c, q, dt in probs]
+            b = Batch(vel, conc, dts, 1)
+            assert b.node_velocity == bool(on)
+            b.step(2)
+            got[on] = [b.coeffs(p) for p in range(len(probs))]
+            b.close()
+        _lib.set_option(_lib.OPT_NODE_VELOCITY, 1)
+        for (v, c, q, dt), tv in zip(probs, vel):
+            t = tb.Tree(c[0], c[1], q, 1, c[2])
+            for _ in range(2):
+                tb.semilag_insitu(tv, t, 1, dt, 1, 1, "fp32")
+            refs.append(t.get_coeffs())
+    finally:
+        _lib.set_option(_lib.OPT_NODE_VELOCITY, old)
+    for p in range(len(probs)):
+        scale = np.abs(refs[p]).max()
+        assert np.abs(got[1][p] - refs[p]).max() <= 1e-6 * scale, p
+        assert np.abs(got[1][p] - got[0][p]).max() <= 2e-6 * scale, p
 
 
 @pytest.mark.parametrize("q,dof,bc", [(8, 16, 0), (6, 48, 1), (5, 80, 0), (14, 32, 1)])
