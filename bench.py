@@ -132,7 +132,8 @@ def run_mode(mode, vel, anchor, depth, ccf, steps, warmup):
 
 def host_cpu() -> dict:
     """CPU model and core counts of this host (lscpu), and the CPUs this
-    process may run on."""
+    process may run on (the calling thread's affinity: call it before an
+    OpenMP region binds the thread)."""
     import subprocess
 
     info = {}
@@ -733,6 +734,9 @@ def main():
     ap.add_argument("--contraction-only", action="store_true",
                     help="only the contraction-alone measurement on the benchmark tree")
     args = ap.parse_args()
+    # before any OpenMP region: with OMP_PROC_BIND set, the oracle's first
+    # parallel region binds this (the initial) thread to one place
+    cpu_info = host_cpu()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
               file=sys.stderr)
@@ -946,7 +950,7 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_cpu:
         cb = cpu_baseline(anchor, depth, ccf, vcf, args.cpu_frac)
-        cpu = host_cpu()
+        cpu = cpu_info
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or cpu["cpus_allowed"]
         out["cpu_baseline"] = {
             "value": cb["interps_per_s"], "unit": "interps/s", "cores": threads, "kind": "port",
