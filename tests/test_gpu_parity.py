@@ -162,9 +162,11 @@ def test_semilag_rk2_against_oracle(torch, tb, orc):
     assert np.array_equal(got, ref)
 
 
-def test_grid_points_and_refit(torch, tb, orc):
-    anchor, depth = random_tree(12, max_depth=4)
-    q, dof = 7, 2
+@pytest.mark.parametrize("q,dof", [(7, 2), (14, 3), (16, 2)])
+def test_grid_points_and_refit(torch, tb, orc, q, dof):
+    # q = 7: four rows per refit workgroup, groups spanning leaves; q = 14:
+    # one row, values staged through LDS; q = 16: one row, loaded per thread
+    anchor, depth = random_tree(12, max_depth=4 if q < 10 else 2)
     g, o = pair(tb, orc, anchor, depth, q, dof, None)
     gp = g.grid_points().cpu().numpy()
     assert np.array_equal(gp, orc.collect_grid_points(o))
