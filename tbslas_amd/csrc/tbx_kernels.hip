@@ -184,7 +184,8 @@ __device__ __forceinline__ void arrival_node(const StepArgs &a, int64_t i0, int 
 }
 
 // The same step with the workgroup's points regrouped by leaf before each
-// of the 2 num_rk_step + 1 evaluations (regroup_pos in tbx_device.h).  The
+// of the 2 num_rk_step + 1 evaluations (regroup_pos in tbx_device.h; the
+// concentration's only in some modes, see below).  The
 // thread that holds a point after a regroup carries it on: the RK2 state
 // (start point, wrapped midpoint) travels with the point through LDS, and
 // the leaf found for the regroup is the one the evaluation uses.  Lanes
@@ -287,7 +288,14 @@ __device__ __forceinline__ void k_step_rg_body(const VelSource &vel, const TreeV
     }
     lf = locate(conc, st[0], st[1], st[2]);
     double p3[3] = {st[0], st[1], st[2]};
-    regroup(sh, orig, lf, p3);
+    // A departure point lies mostly in its midpoint's leaf, by which the
+    // points were just regrouped.  Regrouping again before a one-component
+    // contraction costs the FP32 + FP64 head and FP64 kernels more than the
+    // leaf passes it saves (headline kernel 25.3 against 26.1 ms at q = 14);
+    // the FP32 kernel at its 64-register budget, and many components on the
+    // matrix cores, keep it.
+    constexpr bool kConcRegroup = MF || !(MODE == kFp32h || MODE == kFma64);
+    if constexpr (kConcRegroup) regroup(sh, orig, lf, p3);
     const int64_t gi = i0 + orig;
     const bool live = gi < a.npts;
     if (live && dep) {
