@@ -183,13 +183,15 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
   constexpr int TS = R > 1 ? D * D : kRefitThreads;  // threads of a row's team
   __shared__ double sa[R * P];
   __shared__ double sb[R * P];
-  __shared__ double sT[D * D];  // sT[k * D + a] = S[k][a]
   const int tid = threadIdx.x;
   const int rl = tid / TS < R ? tid / TS : R - 1;  // this thread's row of the group
   const int tl = tid - (tid / TS) * TS;
   const bool team = tid < R * TS;
   double *ra = sa + rl * P, *rb = sb + rl * P;
-  for (int t = tid; t < D * D; t += kRefitThreads) sT[t] = tab.T[t / D][t % D];
+  // The one-axis table S is read from the kernel arguments: every index
+  // into it is wave-uniform, so its entries arrive as scalar operands
+  // instead of LDS broadcasts (k_refit_expand keeps its copy in LDS; read
+  // this way there it ran slower).
   // The first (c, b) line of the thread's next row is loaded while the
   // current row goes through its LDS stages, so that each row does not
   // wait for its own loads (two workgroups per CU at q = 14).
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
         auto out = [&](int k) {
           double s = 0.0;
 #pragma unroll
-          for (int a = 0; a < D; ++a) s = __builtin_fma(sT[k * D + a], x[a], s);
+          for (int a = 0; a < D; ++a) s = __builtin_fma(tab.T[k][a], x[a], s);
           ra[t * D + k] = s;
         };
         // several rows at once: one output at a time keeps the registers
@@ -247,7 +249,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
         for (int j = 0; j + k < D; ++j) {
           double s = 0.0;
 #pragma unroll
-          for (int b = 0; b < D; ++b) s = __builtin_fma(sT[j * D + b], x[b], s);
+          for (int b = 0; b < D; ++b) s = __builtin_fma(tab.T[j][b], x[b], s);
           rb[(c * D + j) * D + k] = s;
         }
       }
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_project(const double *_
         for (int i = 0; i + j + k < D; ++i) {
           double s = 0.0;
 #pragma unroll
-          for (int c = 0; c < D; ++c) s = __builtin_fma(sT[i * D + c], x[c], s);
+          for (int c = 0; c < D; ++c) s = __builtin_fma(tab.T[i][c], x[c], s);
           if constexpr (ACC)
             w[tri_index<D>(i, j, k)] += s;
           else
