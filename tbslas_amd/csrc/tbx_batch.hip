@@ -36,7 +36,7 @@ __device__ __forceinline__ int find_problem(const BatchProblem *pb, int nprob, i
 }
 
 // One item of degree D - 1: leaves [leaf0, leaf0 + nleaf) of problem p.
-template <int D, int BLOCK>
+template <int D, int BLOCK, int LPI>
 __device__ __forceinline__ void batch_item(const BatchProblem &pb, int parity, int64_t leaf0,
                                            int nleaf, float *vals, RegroupLds<BLOCK, 6> &sh) {
   constexpr int P = D * D * D;
@@ -123,15 +123,27 @@ __device__ __forceinline__ void batch_item(const BatchProblem &pb, int parity, i
   }
   __syncthreads();
   // refit of the item's leaves: c'[b][m] = sum_p M[p][m] v[b][p], summed in
-  // FP64 (a sum of P terms; the contraction above is the FP32 part)
+  // FP64 (a sum of P terms; the contraction above is the FP32 part).  One
+  // thread per m does every leaf of the item, so each entry of M (up to
+  // 3 MB, streamed from L2) is loaded once per item, not once per leaf.
+  // The loop has no branch, so the unrolled loads of M stay in flight
+  // together; the sums of the item's missing leaves (b >= nleaf) run on
+  // whatever the LDS holds there and are not stored.
   float *out = pb.out32[parity];
-  for (int t = threadIdx.x; t < nleaf * NC; t += BLOCK) {
-    const int b = t / NC, mm = t - b * NC;
-    const float *vb = vals + b * P;
-    double acc = 0.0;
-#pragma unroll 4
-    for (int p = 0; p < P; ++p) acc = __builtin_fma(pb.M[p * NC + mm], (double)vb[p], acc);
-    out[(leaf0 + b) * NC + mm] = (float)acc;
+  const double *__restrict__ M = pb.M;
+  for (int mm = threadIdx.x; mm < NC; mm += BLOCK) {
+    double acc[LPI];
+#pragma unroll
+    for (int b = 0; b < LPI; ++b) acc[b] = 0.0;
+#pragma unroll 8
+    for (int p = 0; p < P; ++p) {
+      const double m = M[p * NC + mm];
+#pragma unroll
+      for (int b = 0; b < LPI; ++b) acc[b] = __builtin_fma(m, (double)vals[b * P + p], acc[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < LPI; ++b)
+      if (b < nleaf) out[(leaf0 + b) * NC + mm] = (float)acc[b];
   }
   __syncthreads();
 }
@@ -155,13 +167,13 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const BatchProblem *__restrict_
     const int nleaf = (int)(b.L - leaf0 < LPI ? b.L - leaf0 : LPI);
     switch (b.q) {
       case 6:
-        batch_item<7, BLOCK>(b, parity, leaf0, nleaf, vals, sh);
+        batch_item<7, BLOCK, LPI>(b, parity, leaf0, nleaf, vals, sh);
         break;
       case 8:
-        batch_item<9, BLOCK>(b, parity, leaf0, nleaf, vals, sh);
+        batch_item<9, BLOCK, LPI>(b, parity, leaf0, nleaf, vals, sh);
         break;
       default:
-        batch_item<11, BLOCK>(b, parity, leaf0, nleaf, vals, sh);
+        batch_item<11, BLOCK, LPI>(b, parity, leaf0, nleaf, vals, sh);
         break;
     }
   }
