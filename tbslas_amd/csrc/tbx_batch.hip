@@ -135,7 +135,7 @@ __device__ __forceinline__ void batch_item(const BatchProblem &pb, int parity, i
     double acc[LPI];
 #pragma unroll
     for (int b = 0; b < LPI; ++b) acc[b] = 0.0;
-#pragma unroll 8
+#pragma unroll 16
     for (int p = 0; p < P; ++p) {
       const double m = M[p * NC + mm];
 #pragma unroll
