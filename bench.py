@@ -40,7 +40,7 @@ from tbslas_amd import workloads as W  # noqa: E402
 Q = W.ADAPTIVE_Q
 DT = W.ADAPTIVE_DT
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X published dense
-# This build issues no packed FP32 instructions, so an FP32 vector kernel
+# The kernels issue no packed FP32 instructions, so an FP32 vector kernel
 # issues one FMA per lane per clock, the FP64 rate (64 FMA/clock/CU).
 PEAK_FP32_UNPACKED = 78.6
 MFMA_PEAK_F64, MFMA_PEAK_F32 = PEAK["fp64"], PEAK["fp32"]  # dense matrix cores
@@ -738,7 +738,7 @@ def main():
     # parallel region binds this (the initial) thread to one place
     cpu_info = host_cpu()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it",
+        print("bench.py: measures one GPU; --gpus above 1 is not supported",
               file=sys.stderr)
         sys.exit(2)
     # the CPU baseline's thread placement (set before the oracle's OpenMP
@@ -873,7 +873,7 @@ def main():
              "points_per_launch": N, "timing": "HIP events on the launch stream"}
         if mode != "fp64":
             r["peak_note"] = ("published FP32 vector peak, reached only by packed FP32, which "
-                              "this build does not issue")
+                              "these kernels do not issue")
             r["frac_of_unpacked_fp32"] = ach / PEAK_FP32_UNPACKED
         if _lib.get_option(_lib.OPT_NODE_VELOCITY):
             # achieved counts the reference's flops; the kernel executes fewer

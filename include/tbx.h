@@ -50,7 +50,7 @@ enum {
   TBX_ECOVER = 4,      /* leaves overlap or leave a gap in [0,1)^3       */
   TBX_EHIP = 5,        /* HIP runtime or library error                  */
   TBX_EOVERFLOW = 6,   /* size exceeds an int64/int32 limit             */
-  TBX_EUNSUPPORTED = 7 /* degree or combination this build does not cover */
+  TBX_EUNSUPPORTED = 7 /* degree or combination the library does not cover */
 };
 
 enum { TBX_BC_FREESPACE = 0, TBX_BC_PERIODIC = 1 }; /* sim_config.h:59 bc */
